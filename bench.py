@@ -1,0 +1,180 @@
+"""BSSN evolution throughput on one MI355X: gridpoint-RK4-steps/s, FP64.
+
+Workload (default): full-3D BSSN on a SinhSpherical 192x96x96 grid (AMPL=64,
+SINHW=0.2), 6th-order finite differences with Kreiss-Oliger dissipation 0.3,
+1+log lapse and Gamma-driver shift (eta=2), RK4 at CFL 0.5, Brill-Lindquist
+data with masses 0.6 and 0.4 at (0.3,0.2,0.5) and (-0.3,-0.2,-0.5).
+
+value = interior points x timed RK4 steps / seconds of MoL_step_forward_in_time,
+with all fields resident in HBM.  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps 100] [--warmup 5] [--config c3|c2] [--no-cpu]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+CONFIGS = {
+    "c3": dict(workload="bssn_sinhspherical_192x96x96_fd6_ko0.3_rk4", coord=2, Nxx=(192, 96, 96), fd_order=6,
+               sym=0, KO=True, size=64.0, sinhw=0.2, masses=(0.6, 0.4), pos1=(0.3, 0.2, 0.5),
+               pos2=(-0.3, -0.2, -0.5), variant=1),
+    "c2": dict(workload="bssn_spherical_72x12x2_fd4_rk4", coord=1, Nxx=(72, 12, 2), fd_order=4, sym=4, KO=False,
+               size=7.5, sinhw=0.2, masses=(0.5, 0.5), pos1=(0, 0, 0.5), pos2=(0, 0, -0.5), variant=0),
+}
+PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector (= FP64 matrix) peak, dense
+PEAK_HBM_GBS = 8000.0
+
+
+def cpu_baseline(c: dict, steps: int = 1) -> dict:
+    """The oracle (plain-C restatement) on the host cores, one RK4 step of the same grid."""
+    import numpy as np
+
+    from nrpy_amd import initial_data as ID
+    from oracle import oracle
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", min(16, os.cpu_count() or 1)))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    O = oracle.OracleGrid(c["coord"], c["Nxx"], c["fd_order"], c["size"], c["sinhw"], variant=c["variant"])
+    y = ID.brill_lindquist(c["coord"], O.xx, c["size"], c["sinhw"], c["masses"], c["pos1"], c["pos2"])
+    O.apply_bcs_outerextrap_and_inner(y)
+    st = O.new_state(np.ascontiguousarray(y))
+    dt = O.cfl_dt(0.5)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        O.bssn_mol_step(st, O.params(eta=2.0), dt)
+    sec = time.perf_counter() - t0
+    npts = c["Nxx"][0] * c["Nxx"][1] * c["Nxx"][2]
+    return {
+        "value": npts * steps / sec,
+        "unit": "gridpoint-RK4-steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{steps} RK4 step(s) of the full {c['Nxx'][0]}x{c['Nxx'][1]}x{c['Nxx'][2]} grid, "
+                  f"plain-C restatement (OpenMP, {threads} threads), {sec:.1f} s",
+    }
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the single-patch path runs on one device; multi-GPU sharding is design-only).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from nrpy_amd import initial_data as ID
+    from nrpy_amd.runtime import MoLState, Patch, PatchConfig
+
+    c = CONFIGS[args.config]
+    torch.cuda.set_device(0)
+    cfg = PatchConfig(coord=c["coord"], Nxx=c["Nxx"], fd_order=c["fd_order"], symmetry_axes=c["sym"],
+                      enable_KO=c["KO"], grid_physical_size=c["size"], SINHW=c["sinhw"], eta=2.0,
+                      KO_strength_gauge=0.3, KO_strength_nongauge=0.3)
+    P = Patch(cfg)
+    xx = P.xx()
+    y = ID.brill_lindquist(c["coord"], xx, c["size"], c["sinhw"], c["masses"], c["pos1"], c["pos2"])
+    S = MoLState(P, torch.from_numpy(np.ascontiguousarray(y)).cuda())
+    P.apply_bcs_outerextrap_and_inner(S.y_n)
+    dt = P.dt
+    use_graph = not args.no_graph
+    if args.warmup:
+        S.steps(args.warmup, dt, use_graph=use_graph)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    S.steps(args.steps, dt, use_graph=use_graph)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = ev0.elapsed_time(ev1)
+    sec = max(wall, gpu_ms / 1e3)
+    npts = c["Nxx"][0] * c["Nxx"][1] * c["Nxx"][2]
+    value = npts * args.steps / sec
+    assert torch.isfinite(S.y_n).all().item(), "evolution produced non-finite values"
+
+    # Dominant kernel (rhs_eval) timed on its own stream with HIP events.
+    with open(os.path.join(ROOT, "nrpy_amd", "csrc", "gen", "kernel_meta.json")) as f:
+        meta = json.load(f)["flops_per_point"]
+    tag = {1: "Spherical_sym2_KO0", 2: "SinhSpherical_symnone_KO1"}[c["coord"]]
+    flops_rhs = meta[f"bssn_rhs_{tag}_fd{c['fd_order']}"]
+    flops_ricci = meta[f"ricci_{tag}_fd{c['fd_order']}"]
+    reps = 10
+    P.Ricci_eval(S.y_n, S.aux)
+    P.rhs_eval(S.aux, S.y_n, S.k_odd)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        P.rhs_eval(S.aux, S.y_n, S.k_odd)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    rhs_ms = e0.elapsed_time(e1) / reps
+    e0.record(stream)
+    for _ in range(reps):
+        P.Ricci_eval(S.y_n, S.aux)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ricci_ms = e0.elapsed_time(e1) / reps
+    achieved = flops_rhs * npts / (rhs_ms * 1e-3) / 1e12
+    result = {
+        "metric": "BSSN gridpoint-RK4-steps/s FP64 on 1 GPU",
+        "value": value,
+        "unit": "gridpoint-RK4-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (Brill-Lindquist initial data computed in-process)",
+        "config": {
+            "workload": c["workload"],
+            "grid": list(c["Nxx"]),
+            "interior_points": npts,
+            "fd_order": c["fd_order"],
+            "dt": dt,
+            "hipgraph": use_graph,
+        },
+        "roofline": {
+            "bound": "mfma",
+            "compute": "FP64 VALU (stencil arithmetic, no MFMA); peak is MI355X dense FP64",
+            "kernel": f"bssn_rhs_{tag}_fd{c['fd_order']}",
+            "achieved": achieved,
+            "peak": PEAK_FP64_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP64_TFLOPS,
+            "traffic": None,
+            "flops_per_point": flops_rhs,
+            "avg_launch_ms": rhs_ms,
+            "ricci_avg_launch_ms": ricci_ms,
+            "ricci_tflops": flops_ricci * npts / (ricci_ms * 1e-3) / 1e12,
+        },
+    }
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(c)
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,173 @@
+/*
+ * bhah_hip.h -- C-ABI of the MI355X (gfx950) evolution backend for NRPy's
+ * BlackHoles@Home (BHaH) single-patch infrastructure.
+ *
+ * Every entry point is the counterpart of a C function that NRPy generates
+ * for a BHaH project (file:line of the reference emitter or its frozen
+ * output are cited per function).  The reference's functions take
+ * project-generated structs (commondata_struct, params_struct, rfm_struct,
+ * bc_struct, griddata_struct) whose layouts differ per project; this ABI
+ * replaces them with one opaque patch handle built from a plain descriptor.
+ * A generated BHaH project binds these through a thin adapter (see the
+ * integration notes shipped with the repository).
+ *
+ * Conventions
+ *  - All REAL arrays are FP64 device pointers (hipMalloc'd or torch tensors),
+ *    gf-major SoA with IDX4(gf,i0,i1,i2) = i0 + Nx0*(i1 + Nx1*(i2 + Nx2*gf)),
+ *    Nxi = Nxxi + 2*NGHOSTS, xx0 fastest -- the BHaH layout
+ *    (nrpy/infrastructures/BHaH/tests/BHaH_defines_h_BHaH_defines.c:149-162).
+ *  - Evolved gridfunction order (24): aDD00..aDD22, alpha, betU0..2, cf,
+ *    hDD00..hDD22, lambdaU0..2, trK, vetU0..2 (nrpy/grid.py:248-251);
+ *    auxevol: RbarDD00..RbarDD22 (6).
+ *  - Work is enqueued on the patch's stream; calls return without
+ *    synchronising unless stated.
+ *  - Every function returns 0 on success, a negative BHAH_ERR_* code
+ *    otherwise; bhah_hip_last_error() describes the last failure.  (The
+ *    reference functions are void and abort on error.)
+ */
+#ifndef BHAH_HIP_H
+#define BHAH_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BHAH_HIP_ABI_VERSION 1
+
+enum {
+  BHAH_OK = 0,
+  BHAH_ERR_ARG = -1,
+  BHAH_ERR_HIP = -2,
+  BHAH_ERR_UNSUPPORTED = -3,
+  BHAH_ERR_BCSTRUCT = -4,
+};
+
+enum { BHAH_CARTESIAN = 0, BHAH_SPHERICAL = 1, BHAH_SINHSPHERICAL = 2 };
+enum { BHAH_SYSTEM_BSSN = 0, BHAH_SYSTEM_WAVE = 1 };
+enum { BHAH_OUTER_BC_RADIATION = 0, BHAH_OUTER_BC_EXTRAPOLATION = 1 };
+
+#define BHAH_NUM_EVOL_GFS_BSSN 24
+#define BHAH_NUM_AUXEVOL_GFS_BSSN 6
+#define BHAH_NUM_EVOL_GFS_WAVE 2
+
+/* Plain descriptor replacing commondata_struct/params_struct for one patch. */
+typedef struct bhah_patch_desc {
+  int system;                /* BHAH_SYSTEM_BSSN or BHAH_SYSTEM_WAVE */
+  int coord;                 /* BHAH_CARTESIAN, BHAH_SPHERICAL, BHAH_SINHSPHERICAL */
+  int Nxx[3];                /* interior points per direction */
+  int fd_order;              /* 4 or 6 */
+  int symmetry_axes;         /* bit d set: derivatives along xx_d vanish (e.g. 4 = phi) */
+  int enable_KO;             /* Kreiss-Oliger dissipation on (BSSN) */
+  int outer_bc;              /* BHAH_OUTER_BC_* (BSSN) */
+  int radiation_bc_fd_order; /* FD order of d/dr in the radiation BC (reference default 4) */
+  double grid_physical_size; /* RMAX (Spherical), AMPL (SinhSpherical), half-width (Cartesian) */
+  double SINHW;              /* SinhSpherical width */
+  double CFL_FACTOR;         /* dt = CFL_FACTOR * min ds */
+  double eta;                /* Gamma-driver damping */
+  double KO_strength_gauge, KO_strength_nongauge;
+  double wavespeed;          /* scalar wave speed */
+} bhah_patch_desc;
+
+typedef struct bhah_patch bhah_patch;
+
+/* MoL gridfunction arrays (MoL_gridfunctions_struct,
+ * nrpy/infrastructures/BHaH/MoLtimestepping/MoL_malloc_intermediate_stage_gfs.py:20). */
+typedef struct bhah_MoL_gridfunctions {
+  double *y_n_gfs;
+  double *y_nplus1_running_total_gfs;
+  double *k_odd_gfs;
+  double *k_even_gfs;
+  double *auxevol_gfs;
+} bhah_MoL_gridfunctions;
+
+/* Grid geometry of a created patch (numerical_grid_params_Nxx_dxx_xx). */
+typedef struct bhah_patch_info {
+  int Nxx[3], NGHOSTS, Nxx_plus_2NGHOSTS[3];
+  int num_evol_gfs, num_auxevol_gfs;
+  int64_t Ntot;       /* points per gridfunction, ghost zones included */
+  double xxmin[3], xxmax[3], dxx[3], invdxx[3];
+  double dt;          /* CFL-limited timestep */
+  int num_inner_boundary_points;
+  int num_pure_outer_boundary_points;
+} bhah_patch_info;
+
+const char *bhah_hip_last_error(void);
+int bhah_hip_abi_version(void);
+
+/* Set up grid, reference-metric precompute arrays, boundary structs and CFL
+ * timestep on the current HIP device.  Replaces numerical_grids_and_timestep
+ * (nrpy/infrastructures/BHaH/numerical_grids_and_timestep.py:31-760),
+ * rfm_precompute (nrpy/infrastructures/BHaH/rfm_precompute.py:84-400) and
+ * bcstruct_set_up (nrpy/infrastructures/BHaH/CurviBoundaryConditions/bcstruct_set_up.py:38-792). */
+int bhah_hip_patch_create(const bhah_patch_desc *desc, bhah_patch **out);
+int bhah_hip_patch_destroy(bhah_patch *patch);
+int bhah_hip_patch_get_info(const bhah_patch *patch, bhah_patch_info *info);
+/* Use an existing hipStream_t (e.g. torch.cuda.current_stream().cuda_stream). */
+int bhah_hip_patch_set_stream(bhah_patch *patch, void *hip_stream);
+int bhah_hip_synchronize(bhah_patch *patch);
+/* Host copies of coordinates (3 arrays of Nxx_plus_2NGHOSTS[d]) and, for
+ * tests, of the boundary structs. */
+int bhah_hip_patch_get_xx(const bhah_patch *patch, double *xx0, double *xx1, double *xx2);
+int bhah_hip_patch_get_inner_bc(const bhah_patch *patch, int64_t *dstpt, int64_t *srcpt, int8_t *parity10);
+int bhah_hip_patch_get_outer_bc_count(const bhah_patch *patch, int which_gz, int dirn);
+int bhah_hip_patch_get_outer_bc(const bhah_patch *patch, int which_gz, int dirn, int16_t *i0i1i2_face6);
+
+/* rhs_eval (nrpy/infrastructures/BHaH/general_relativity/rhs_eval.py:97-120; call
+ * nrpy/examples/two_blackholes_collide.py:294): BSSN right-hand sides at interior points. */
+int bhah_hip_rhs_eval(bhah_patch *patch, const double *auxevol_gfs, const double *in_gfs, double *rhs_gfs);
+
+/* Ricci_eval (nrpy/infrastructures/BHaH/general_relativity/Ricci_eval.py:81-93; call
+ * two_blackholes_collide.py:292): RbarDD into auxevol_gfs at interior points. */
+int bhah_hip_Ricci_eval(bhah_patch *patch, const double *in_gfs, double *auxevol_gfs);
+
+/* enforce_detgammabar_equals_detgammahat (…/enforce_detgammabar_equals_detgammahat.py:63-83;
+ * call two_blackholes_collide.py:307): all points, in place. */
+int bhah_hip_enforce_detgammabar_equals_detgammahat(bhah_patch *patch, double *in_gfs);
+
+/* apply_bcs_outerradiation_and_inner (CurviBoundaryConditions/tests/
+ * apply_bcs_outerradiation_and_inner__rfm__openmp__HoleySinhSpherical.c:211-237):
+ * Sommerfeld RHS at pure outer points, then inner parity on rhs_gfs. */
+int bhah_hip_apply_bcs_outerradiation_and_inner(bhah_patch *patch, const double *gfs, double *rhs_gfs);
+
+/* apply_bcs_outerextrap_and_inner (…/apply_bcs_outerextrap_and_inner__openmp.c:80-91). */
+int bhah_hip_apply_bcs_outerextrap_and_inner(bhah_patch *patch, double *gfs);
+
+/* apply_bcs_inner_only (…/apply_bcs_inner_only_apply_bcs_inner_only__openmp.c:58-70). */
+int bhah_hip_apply_bcs_inner_only(bhah_patch *patch, double *gfs);
+
+/* MoL_malloc_intermediate_stage_gfs (MoLtimestepping/MoL_malloc_intermediate_stage_gfs.py:20):
+ * allocates y_nplus1_running_total, k_odd, k_even and auxevol on the device; y_n_gfs is
+ * allocated too when it is NULL. */
+int bhah_hip_MoL_malloc_intermediate_stage_gfs(bhah_patch *patch, bhah_MoL_gridfunctions *gf);
+int bhah_hip_MoL_free(bhah_patch *patch, bhah_MoL_gridfunctions *gf);
+
+/* MoL_step_forward_in_time, RK4 (MoLtimestepping/MoL_step_forward_in_time.py:39-507; frozen
+ * MoL_step_forward_in_time_CUDA__MoL_step_forward_in_time__RK4.cu:182-287): one full step of
+ * size dt.  *time and *nn follow the reference's update (time = t_0 + (nn - nn_0 + 1) dt). */
+int bhah_hip_MoL_step_forward_in_time(bhah_patch *patch, bhah_MoL_gridfunctions *gf, double dt, double *time);
+/* nsteps steps; with use_graph != 0 the four substeps are captured once in a hipGraph and
+ * replayed (no HIP API work per step beyond the graph launch). */
+int bhah_hip_MoL_step_n(bhah_patch *patch, bhah_MoL_gridfunctions *gf, double dt, int nsteps, int use_graph);
+
+/* Wave equation: rhs_eval (nrpy/examples/wave_equation_cartesian.py:318-357) and quadratic
+ * extrapolation apply_bcs (wave_equation_cartesian.py:374-431). */
+int bhah_hip_wave_rhs_eval(bhah_patch *patch, const double *in_gfs, double *rhs_gfs);
+int bhah_hip_wave_apply_bcs(bhah_patch *patch, double *gfs);
+
+/* Diagnostics (nrpy/infrastructures/BHaH/diagnostics/diagnostics.py:46-270; constraints
+ * nrpy/equations/general_relativity/BSSN_constraints.py:82-157; volume integrals
+ * diagnostics_volume_integration_helpers.h:396-476): Hamiltonian constraint H and momentum
+ * constraint MU^i at interior points into diag (device, 4*Ntot: H, MU0, MU1, MU2).  If l2_H is
+ * not NULL it receives {RMS of H over the proper volume, sqrt(integral of H^2 dV), volume} with
+ * dV = sqrt(det ghat) dxx0 dxx1 dxx2 (synchronises the stream).  Needs RbarDD in auxevol_gfs
+ * (call bhah_hip_Ricci_eval on in_gfs first). */
+int bhah_hip_diagnostics_constraints(bhah_patch *patch, const double *auxevol_gfs, const double *in_gfs, double *diag,
+                                     double *l2_H);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BHAH_HIP_H */

@@ -1,0 +1,561 @@
+"""Emit gfx950 HIP kernels for the BSSN and scalar-wave evolution path.
+
+Kernels are generated from this repository's SymPy transcription
+(``nrpy_amd.symbolic``) with FP64 arithmetic, no MFMA (stencil work is not a
+dense contraction).  One thread owns one gridpoint; a 64-lane wavefront spans
+64 consecutive xx0 points, so every stencil load of a wavefront is one
+coalesced 512-byte row.  Design choices, per kernel family:
+
+* Interior BSSN RHS (``rhs_eval``, ``nrpy/infrastructures/BHaH/general_relativity/rhs_eval.py:336-379``):
+  centre values, stencil sums and the CSE'd algebra are emitted in
+  first-use order so the register allocator sees short live ranges.
+  Upwinded derivatives use one weight vector per direction, selected once per
+  point from the sign of beta^i and shared by all gridfunctions, instead of two
+  one-sided stencils blended afterwards (``nrpy/c_codegen.py:1085-1150``); the
+  result is the same derivative up to rounding.
+* Fused epilogue: the RK4 substep update
+  (``MoLtimestepping/tests/register_all_MoL_step_forward_in_time.c:4-103``) and
+  det(gammabar) enforcement (``enforce_detgammabar_equals_detgammahat.py:84-98``)
+  are applied in registers before the single store of the next stage; the raw
+  RHS is kept only at points that boundary conditions read.
+* Ricci (``Ricci_eval.py:95-142``) is a separate interior kernel writing the 6
+  auxiliary gridfunctions.
+"""
+
+from __future__ import annotations
+
+import os
+from typing import Dict, List, Set, Tuple
+
+import sympy as sp
+
+from ..symbolic import bssn
+from . import emit, fd, specs
+
+RFM_EXPR = {
+    # rfm slot -> C expression reading the 1D precompute arrays
+    0: "rfm0[i0]",
+    1: "rfm0[p.Nx0 + i0]",
+    2: "rfm0[2 * p.Nx0 + i0]",
+    3: "rfm0[3 * p.Nx0 + i0]",
+    4: "rfm1[i1]",
+    5: "rfm1[p.Nx1 + i1]",
+    6: "rfm1[2 * p.Nx1 + i1]",
+}
+PARAM_EXPR = {
+    "eta": "p.eta",
+    "KreissOliger_strength_gauge": "p.KO_gauge",
+    "KreissOliger_strength_nongauge": "p.KO_nongauge",
+    "wavespeed": "p.wavespeed",
+}
+
+
+def _dbl(r: sp.Rational) -> str:
+    if r.q == 1:
+        return f"{int(r.p)}.0"
+    return f"({int(r.p)}.0 / {int(r.q)}.0)"
+
+
+class StencilEmitter:
+    """C expressions for derivative symbols, reading ``in`` around ``idx``."""
+
+    def __init__(self, spec: specs.Spec, fd_order: int, array: str = "in"):
+        self.spec = spec
+        self.fd_order = fd_order
+        self.array = array
+        self.R = fd.ghost_zones(fd_order)
+
+    def load(self, gf: str, off: Tuple[int, int, int]) -> str:
+        g = self.spec.gfs.index(gf)
+        terms = []
+        if off[0]:
+            terms.append(f"{off[0]}")
+        if off[1]:
+            terms.append(f"{off[1]} * s1")
+        if off[2]:
+            terms.append(f"{off[2]} * s2")
+        o = " + ".join(terms) if terms else "0"
+        base = f"{g} * Ntot + idx" if g else "idx"
+        if o == "0":
+            return f"{self.array}[{base}]"
+        return f"{self.array}[{base} + ({o})]"
+
+    def _pt(self, d: int, k: int) -> Tuple[int, int, int]:
+        o = [0, 0, 0]
+        o[d] = k
+        return tuple(o)
+
+    def expr(self, dv: fd.Deriv) -> str:
+        gf, op = dv.gf, dv.op
+        inv = [f"p.invdxx{d}" for d in range(3)]
+        if op == "dD":
+            d = dv.dirs[0]
+            w = dict(fd.weights_1d("dD", self.fd_order))
+            parts = []
+            for k in sorted(o for o in w if o > 0):
+                parts.append(f"{_dbl(w[k])} * ({self.load(gf, self._pt(d, k))} - {self.load(gf, self._pt(d, -k))})")
+            return f"{inv[d]} * ({' + '.join(parts)})"
+        if op == "dKOD" or (op == "dDD" and dv.dirs[0] == dv.dirs[1]):
+            d = dv.dirs[0]
+            w = dict(fd.weights_1d("dKOD" if op == "dKOD" else "dDD", self.fd_order))
+            parts = [f"{_dbl(w[0])} * {self.load(gf, (0, 0, 0))}"]
+            for k in sorted(o for o in w if o > 0):
+                parts.append(f"{_dbl(w[k])} * ({self.load(gf, self._pt(d, k))} + {self.load(gf, self._pt(d, -k))})")
+            scale = inv[d] if op == "dKOD" else f"{inv[d]} * {inv[d]}"
+            return f"{scale} * ({' + '.join(parts)})"
+        if op == "dDD":
+            d0, d1 = dv.dirs
+            w = dict(fd.weights_1d("dD", self.fd_order))
+            parts = []
+            for a in sorted(o for o in w if o > 0):
+                for b in sorted(o for o in w if o > 0):
+                    def L(sa, sb):
+                        o = [0, 0, 0]
+                        o[d0] += sa * a
+                        o[d1] += sb * b
+                        return self.load(gf, tuple(o))
+                    parts.append(
+                        f"{_dbl(w[a] * w[b])} * (({L(1, 1)} - {L(1, -1)}) - ({L(-1, 1)} - {L(-1, -1)}))"
+                    )
+            return f"{inv[d0]} * {inv[d1]} * ({' + '.join(parts)})"
+        if op == "dupD":
+            d = dv.dirs[0]
+            terms = []
+            for k in range(-self.R, self.R + 1):
+                terms.append(f"uw{d}[{k + self.R}] * {self.load(gf, self._pt(d, k))}")
+            return f"{inv[d]} * ({' + '.join(terms)})"
+        raise ValueError(op)
+
+    def upwind_tables(self) -> str:
+        """Per-direction weight vectors selected by the sign of beta^d."""
+        up = dict(fd.weights_1d("dupD", self.fd_order))
+        dn = dict(fd.weights_1d("ddnD", self.fd_order))
+        lines = []
+        for d in range(3):
+            lines.append(f"  double uw{d}[{2 * self.R + 1}];")
+            for k in range(-self.R, self.R + 1):
+                a = _dbl(sp.Rational(up.get(k, 0)))
+                b = _dbl(sp.Rational(dn.get(k, 0)))
+                lines.append(f"  uw{d}[{k + self.R}] = (betaU{d} > 0.0) ? {a} : {b};")
+        return "\n".join(lines)
+
+
+# Per-point FP64 operation counts of the last emitted body of each spec
+# (add/sub/mul/div = 1, fma counted as its mul + add), filled by _ordered_body.
+FLOPS: Dict[str, int] = {}
+
+
+def stencil_flops(d: fd.Deriv, fd_order: int) -> int:
+    """Operations of one emitted stencil sum (pairs of +-k points grouped)."""
+    R = fd.ghost_zones(fd_order)
+    if d.op == "dD":
+        P = len([o for o, _ in fd.weights_1d("dD", fd_order) if o > 0])
+        return 3 * P
+    if d.op == "dKOD" or (d.op == "dDD" and d.dirs[0] == d.dirs[1]):
+        P = len([o for o, _ in fd.weights_1d("dKOD" if d.op == "dKOD" else "dDD", fd_order) if o > 0])
+        return 3 * P + (1 if d.op == "dKOD" else 2)
+    if d.op == "dDD":
+        P = len([o for o, _ in fd.weights_1d("dD", fd_order) if o > 0])
+        return 5 * P * P + 1
+    if d.op == "dupD":
+        return 4 * R + 2
+    raise ValueError(d.op)
+
+
+def split_linear(spec: specs.Spec):
+    """Split each output into core + sum(coef * s) over upwind/KO derivatives s.
+
+    Every ``dupD`` and ``dKOD`` symbol enters the BSSN right-hand sides
+    linearly (advection beta^k d_k X and dissipation eps ReU_k KO_k X), with a
+    coefficient that depends only on the shift and the reference metric.
+    Returns (cores, {output_gf: [(deriv, coef), ...]}).
+    """
+    _, _, derivs, _, _ = spec.classify()
+    lin = {sp.Symbol(d.symbol, real=True): d for d in derivs if d.op in ("dupD", "dKOD")}
+    cores, terms = [], {}
+    for g, e in spec.outputs:
+        core = e.xreplace({s: 0 for s in lin})
+        cores.append(core)
+        for s, d in lin.items():
+            if s in e.free_symbols:
+                c = sp.diff(e, s)
+                if c.free_symbols & set(lin):
+                    raise ValueError(f"{s} enters {g} nonlinearly")
+                terms.setdefault(g, []).append((d, c))
+    return cores, terms
+
+
+def _emit_defs(spec: specs.Spec, fd_order: int):
+    centre, aux, derivs, rfm, params = spec.classify()
+    st = StencilEmitter(spec, fd_order)
+    defs: Dict[str, str] = {}
+    for n in centre:
+        defs[n] = st.load(n, (0, 0, 0))
+    for n in aux:
+        defs[n] = f"aux[{spec.aux.index(n)} * Ntot + idx]"
+    for d in derivs:
+        defs[d.symbol] = st.expr(d)
+    for n in rfm:
+        defs[n] = RFM_EXPR[specs.RFM_SYMBOLS.index(n)]
+    for n in params:
+        defs[n] = PARAM_EXPR[n]
+    return defs
+
+
+def _ordered_body(spec: specs.Spec, fd_order: int, out_lvalues: List[str], rationals: Dict[str, str],
+                  stream_linear: bool = True) -> str:
+    """Stencils and CSE'd algebra for one pointwise spec.
+
+    With ``stream_linear``, upwind and KO derivatives are consumed as soon as
+    they are formed (accumulated into per-output registers), and only the
+    remaining core algebra is CSE'd; centre values, centred derivatives and
+    CSE temporaries are then emitted in first-use order.  Scheduling barriers
+    after each stencil keep the compiler from hoisting every load to the top.
+    """
+    defs = _emit_defs(spec, fd_order)
+    derivs = {d.symbol: d for d in spec.classify()[2]}
+    pr = emit.DoublePrinter(rationals)
+    emitted: Set[str] = set()
+    lines: List[str] = []
+    flops = [0]
+
+    def need(expr, indent="  "):
+        for s in sorted(expr.free_symbols, key=str):
+            n = str(s)
+            if n in defs and n not in emitted:
+                emitted.add(n)
+                if n in derivs:
+                    flops[0] += stencil_flops(derivs[n], fd_order)
+                lines.append(f"{indent}const double {n} = {defs[n]};")
+                if fd.is_deriv_symbol(n):
+                    lines.append(f"{indent}__builtin_amdgcn_sched_barrier(0);")
+
+    outs = [e for _, e in spec.outputs]
+    acc_names: Dict[str, str] = {}
+    if stream_linear:
+        cores, terms = split_linear(spec)
+        if terms:
+            coefs = sorted({c for tl in terms.values() for _, c in tl}, key=sp.default_sort_key)
+            crep, cred = sp.cse(coefs, symbols=sp.numbered_symbols("lc"), order="none")
+            for s, e in crep:
+                need(e)
+                lines.append(f"  const double {s} = {pr.doprint(e)};")
+            cname = {}
+            for k, (c, e) in enumerate(zip(coefs, cred)):
+                need(e)
+                lines.append(f"  const double lcoef{k} = {pr.doprint(e)};")
+                cname[c] = f"lcoef{k}"
+            flops[0] += sum(emit._flops(e) for _, e in crep) + sum(emit._flops(e) for e in cred)
+            for g, tl in terms.items():
+                acc = f"lin_{g}"
+                acc_names[g] = acc
+                lines.append(f"  double {acc} = 0.0;")
+                for d, c in sorted(tl, key=lambda t: t[0].symbol):
+                    lines.append(f"  {acc} += {cname[c]} * ({defs[d.symbol]});")
+                    flops[0] += 2 + stencil_flops(d, fd_order)
+                lines.append("  __builtin_amdgcn_sched_barrier(0);")
+        outs = cores
+    replaced, reduced = sp.cse(outs, symbols=sp.numbered_symbols("t"), order="none")
+    for s, e in replaced:
+        need(e)
+        lines.append(f"  const double {s} = {pr.doprint(e)};")
+    for (g, _), lv, e in zip(spec.outputs, out_lvalues, reduced):
+        need(e)
+        tail = f" + {acc_names[g]}" if g in acc_names else ""
+        lines.append(f"  {lv} = {pr.doprint(e)}{tail};")
+        flops[0] += emit._flops(e) + (1 if tail else 0)
+    flops[0] += sum(emit._flops(e) for _, e in replaced)
+    FLOPS[f"{spec.name}_fd{fd_order}"] = flops[0]
+    return "\n".join(lines)
+
+
+KPARAMS = r"""
+struct bhah_kparams {
+  int Nx0, Nx1, Nx2, NG;
+  long Ntot, s1, s2;
+  double invdxx0, invdxx1, invdxx2;
+  double eta, KO_gauge, KO_nongauge, wavespeed;
+  double dt;
+  int substep;
+};
+"""
+
+
+def _prologue(interior: bool) -> str:
+    lo = "p.NG" if interior else "0"
+    return f"""  const int i0 = {lo} + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int i1 = {lo} + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const int i2 = {lo} + (int)blockIdx.z;
+  if (i0 >= p.Nx0 - {lo} || i1 >= p.Nx1 - {lo}) return;
+  const long s1 = p.s1, s2 = p.s2, Ntot = p.Ntot;
+  const long idx = i0 + s1 * i1 + s2 * i2;
+"""
+
+
+def _upwind_prologue(spec: specs.Spec, fd_order: int, rationals) -> str:
+    st = StencilEmitter(spec, fd_order)
+    pr = emit.DoublePrinter(rationals)
+    lines = []
+    used = emit.free_symbol_names(spec.upwind)
+    for n in sorted(used):
+        if n in spec.gfs:
+            lines.append(f"  const double up_{n} = {st.load(n, (0, 0, 0))};")
+        elif n in specs.RFM_SYMBOLS:
+            lines.append(f"  const double up_{n} = {RFM_EXPR[specs.RFM_SYMBOLS.index(n)]};")
+    sub = {sp.Symbol(n, real=True): sp.Symbol("up_" + n, real=True) for n in used}
+    for d in range(3):
+        lines.append(f"  const double betaU{d} = {pr.doprint(sp.sympify(spec.upwind[d]).xreplace(sub))};")
+    lines.append(st.upwind_tables())
+    return "\n".join(lines)
+
+
+def detgamma_inline(v: specs.Variant, rationals, src: str) -> str:
+    """Inline det(gammabar) enforcement on hDD values held in ``nxt[]``.
+
+    ``src`` names the array of 24 next-stage values (EVOL order).
+    """
+    spec = specs.detgamma_spec(v)
+    pr = emit.DoublePrinter(rationals)
+    centre, _, _, rfm, _ = spec.classify()
+    lines = ["  {"]
+    for n in centre:
+        lines.append(f"    const double {n} = {src}[{bssn.EVOL_GFS.index(n)}];")
+    for n in rfm:
+        lines.append(f"    const double {n} = {RFM_EXPR[specs.RFM_SYMBOLS.index(n)]};")
+    replaced, reduced = sp.cse([e for _, e in spec.outputs], symbols=sp.numbered_symbols("dg"), order="none")
+    for s, e in replaced:
+        lines.append(f"    const double {s} = {pr.doprint(e)};")
+    outs = []
+    for (gf, _), e in zip(spec.outputs, reduced):
+        lines.append(f"    const double new_{gf} = {pr.doprint(e)};")
+        outs.append(gf)
+    for gf in outs:
+        lines.append(f"    {src}[{bssn.EVOL_GFS.index(gf)}] = new_{gf};")
+    lines.append("  }")
+    return "\n".join(lines)
+
+
+RK_EPILOGUE = r"""
+  // RK4 substep update (register_all_MoL_step_forward_in_time.c:4-103), in registers.
+  const double dt = p.dt;
+  double nxt[NGF];
+  if (p.substep == 1) {
+#pragma unroll
+    for (int g = 0; g < NGF; g++) {
+      const double k = kk[g];
+      y_run[g * Ntot + idx] = (1.0 / 6.0) * dt * k;
+      nxt[g] = (1.0 / 2.0) * dt * k + ctr[g];
+    }
+  } else if (p.substep == 4) {
+#pragma unroll
+    for (int g = 0; g < NGF; g++) {
+      const double k = kk[g];
+      nxt[g] = (1.0 / 6.0) * dt * k + y_n[g * Ntot + idx] + y_run[g * Ntot + idx];
+    }
+  } else {
+    const double a = (p.substep == 2) ? (1.0 / 2.0) : 1.0;
+#pragma unroll
+    for (int g = 0; g < NGF; g++) {
+      const double k = kk[g];
+      y_run[g * Ntot + idx] = (1.0 / 3.0) * dt * k + y_run[g * Ntot + idx];
+      nxt[g] = a * dt * k + y_n[g * Ntot + idx];
+    }
+  }
+"""
+
+
+def bssn_kernels(v: specs.Variant, fd_order: int) -> str:
+    tag = f"{v.tag}_fd{fd_order}"
+    rspec = specs.bssn_rhs_spec(v)
+    rats: Dict[str, str] = {}
+    ngf = len(bssn.EVOL_GFS)
+
+    # ---- plain rhs_eval: interior, writes rhs_gfs
+    lv = [f"rhs[{bssn.EVOL_GFS.index(g)} * Ntot + idx]" for g, _ in rspec.outputs]
+    body = _ordered_body(rspec, fd_order, lv, rats)
+    upw = _upwind_prologue(rspec, fd_order, rats)
+    plain = f"""
+extern "C" __global__ void __launch_bounds__(256)
+bssn_rhs_{tag}(const double *__restrict__ in, const double *__restrict__ aux, double *__restrict__ rhs,
+               const double *__restrict__ rfm0, const double *__restrict__ rfm1, const bhah_kparams p) {{
+{_prologue(True)}
+{upw}
+{body}
+}}
+"""
+    # ---- RK4 substep update + det(gammabar) enforcement, all points, one streaming pass
+    det = detgamma_inline(v, rats, "nxt")
+    rkdet = f"""
+extern "C" __global__ void __launch_bounds__(256)
+bssn_rk_det_{tag}(const double *__restrict__ kin, const double *y_n, double *__restrict__ y_run, double *out,
+                  const double *__restrict__ rfm0, const double *__restrict__ rfm1, const bhah_kparams p) {{
+{_prologue(False)}
+  constexpr int NGF = {ngf};
+  double kk[NGF], ctr[NGF];
+#pragma unroll
+  for (int g = 0; g < NGF; g++) {{
+    kk[g] = kin[g * Ntot + idx];
+    ctr[g] = (p.substep == 1) ? y_n[g * Ntot + idx] : 0.0;
+  }}
+{RK_EPILOGUE}
+{det}
+#pragma unroll
+  for (int g = 0; g < NGF; g++) out[g * Ntot + idx] = nxt[g];
+}}
+"""
+    # ---- Ricci
+    cspec = specs.ricci_spec(v)
+    lv = [f"aux[{bssn.AUXEVOL_GFS.index(g)} * Ntot + idx]" for g, _ in cspec.outputs]
+    body = _ordered_body(cspec, fd_order, lv, rats)
+    ricci = f"""
+extern "C" __global__ void __launch_bounds__(256)
+bssn_ricci_{tag}(const double *__restrict__ in, double *__restrict__ aux,
+                 const double *__restrict__ rfm0, const double *__restrict__ rfm1, const bhah_kparams p) {{
+{_prologue(True)}
+{body}
+}}
+"""
+    # ---- det enforcement over all points (standalone ABI call)
+    dspec = specs.detgamma_spec(v)
+    det_all = f"""
+extern "C" __global__ void __launch_bounds__(256)
+bssn_detgamma_{tag}(double *__restrict__ gfs, const double *__restrict__ rfm0, const double *__restrict__ rfm1,
+                    const bhah_kparams p) {{
+{_prologue(False)}
+  double nxt[{ngf}];
+#pragma unroll
+  for (int g = 0; g < {ngf}; g++) nxt[g] = 0.0;
+{"".join(f"  nxt[{bssn.EVOL_GFS.index(g)}] = gfs[{bssn.EVOL_GFS.index(g)} * Ntot + idx];{chr(10)}" for g, _ in dspec.outputs)}
+{detgamma_inline(v, rats, "nxt")}
+{"".join(f"  gfs[{bssn.EVOL_GFS.index(g)} * Ntot + idx] = nxt[{bssn.EVOL_GFS.index(g)}];{chr(10)}" for g, _ in dspec.outputs)}
+}}
+"""
+    # ---- constraints H, MU^i at interior points
+    kspec = specs.constraints_spec(v)
+    lv = [f"diag[{k} * Ntot + idx]" for k in range(len(kspec.outputs))]
+    body = _ordered_body(kspec, fd_order, lv, rats)
+    cons = f"""
+extern "C" __global__ void __launch_bounds__(256)
+bssn_constraints_{tag}(const double *__restrict__ in, const double *__restrict__ aux, double *__restrict__ diag,
+                       const double *__restrict__ rfm0, const double *__restrict__ rfm1, const bhah_kparams p) {{
+{_prologue(True)}
+{body}
+}}
+"""
+    head = [f"// ---- BSSN kernels: {v.coord}, symmetry axes '{v.symmetry_axes}', KO={v.enable_KO}, fd_order={fd_order}"]
+    consts = "\n".join(f"#define {k} ({v})" for k, v in sorted(rats.items()))
+    undefs = "\n".join(f"#undef {k}" for k in sorted(rats))
+    launch = f"""
+// Host-side launchers (one translation unit per variant; no relocatable device code needed).
+extern "C" hipError_t launch_bssn_rhs_{tag}(dim3 g, dim3 b, hipStream_t s, const double *in, const double *aux,
+    double *rhs, const double *rfm0, const double *rfm1, bhah_kparams p) {{
+  hipLaunchKernelGGL(bssn_rhs_{tag}, g, b, 0, s, in, aux, rhs, rfm0, rfm1, p);
+  return hipGetLastError();
+}}
+extern "C" hipError_t launch_bssn_rk_det_{tag}(dim3 g, dim3 b, hipStream_t s, const double *kin, const double *y_n,
+    double *y_run, double *out, const double *rfm0, const double *rfm1, bhah_kparams p) {{
+  hipLaunchKernelGGL(bssn_rk_det_{tag}, g, b, 0, s, kin, y_n, y_run, out, rfm0, rfm1, p);
+  return hipGetLastError();
+}}
+extern "C" hipError_t launch_bssn_constraints_{tag}(dim3 g, dim3 b, hipStream_t s, const double *in,
+    const double *aux, double *diag, const double *rfm0, const double *rfm1, bhah_kparams p) {{
+  hipLaunchKernelGGL(bssn_constraints_{tag}, g, b, 0, s, in, aux, diag, rfm0, rfm1, p);
+  return hipGetLastError();
+}}
+extern "C" hipError_t launch_bssn_ricci_{tag}(dim3 g, dim3 b, hipStream_t s, const double *in, double *aux,
+    const double *rfm0, const double *rfm1, bhah_kparams p) {{
+  hipLaunchKernelGGL(bssn_ricci_{tag}, g, b, 0, s, in, aux, rfm0, rfm1, p);
+  return hipGetLastError();
+}}
+extern "C" hipError_t launch_bssn_detgamma_{tag}(dim3 g, dim3 b, hipStream_t s, double *gfs, const double *rfm0,
+    const double *rfm1, bhah_kparams p) {{
+  hipLaunchKernelGGL(bssn_detgamma_{tag}, g, b, 0, s, gfs, rfm0, rfm1, p);
+  return hipGetLastError();
+}}
+"""
+    return "\n".join(head) + "\n" + consts + "\n" + plain + rkdet + ricci + det_all + cons + undefs + "\n" + launch
+
+
+def wave_kernels(fd_order: int) -> str:
+    spec = specs.wave_spec()
+    rats: Dict[str, str] = {}
+    tag = f"fd{fd_order}"
+    lv = [f"rhs[{spec.gfs.index(g)} * Ntot + idx]" for g, _ in spec.outputs]
+    body = _ordered_body(spec, fd_order, lv, rats)
+    lvk = [f"kk[{spec.gfs.index(g)}]" for g, _ in spec.outputs]
+    bodyk = _ordered_body(spec, fd_order, lvk, rats)
+    consts = "\n".join(f"#define {k} ({v})" for k, v in sorted(rats.items()))
+    undefs = "\n".join(f"#undef {k}" for k in sorted(rats))
+    return f"""// ---- scalar wave kernels, Cartesian, fd_order={fd_order}
+{consts}
+extern "C" __global__ void __launch_bounds__(256)
+wave_rhs_{tag}(const double *__restrict__ in, double *__restrict__ rhs, const bhah_kparams p) {{
+{_prologue(True)}
+{body}
+}}
+
+extern "C" __global__ void __launch_bounds__(256)
+wave_rhs_rk_{tag}(const double *__restrict__ in, const double *y_n, double *__restrict__ y_run,
+                  double *out, const bhah_kparams p) {{
+{_prologue(True)}
+  constexpr int NGF = 2;
+  double kk[NGF];
+{bodyk}
+  double ctr[NGF];
+#pragma unroll
+  for (int g = 0; g < NGF; g++) ctr[g] = (p.substep == 1) ? in[g * Ntot + idx] : 0.0;
+{RK_EPILOGUE}
+#pragma unroll
+  for (int g = 0; g < NGF; g++) out[g * Ntot + idx] = nxt[g];
+}}
+{undefs}
+
+extern "C" hipError_t launch_wave_rhs_{tag}(dim3 g, dim3 b, hipStream_t s, const double *in, double *rhs, bhah_kparams p) {{
+  hipLaunchKernelGGL(wave_rhs_{tag}, g, b, 0, s, in, rhs, p);
+  return hipGetLastError();
+}}
+extern "C" hipError_t launch_wave_rhs_rk_{tag}(dim3 g, dim3 b, hipStream_t s, const double *in, const double *y_n,
+    double *y_run, double *out, bhah_kparams p) {{
+  hipLaunchKernelGGL(wave_rhs_rk_{tag}, g, b, 0, s, in, y_n, y_run, out, p);
+  return hipGetLastError();
+}}
+"""
+
+
+BSSN_BUILDS = [
+    (specs.VARIANTS[0], 4),
+    (specs.VARIANTS[1], 6),
+    (specs.VARIANTS[1], 4),
+]
+WAVE_BUILDS = [4]
+
+
+def kernel_name_tag(v: specs.Variant, fd_order: int) -> str:
+    return f"{v.tag}_fd{fd_order}"
+
+
+def generate(outdir: str) -> List[str]:
+    os.makedirs(outdir, exist_ok=True)
+    paths = []
+    hdr = "// Generated by nrpy_amd/codegen/hip_gen.py -- do not edit.\n#include <hip/hip_runtime.h>\n#include <math.h>\n" + KPARAMS
+    for v, order in BSSN_BUILDS:
+        path = os.path.join(outdir, f"bssn_{kernel_name_tag(v, order)}.hip")
+        with open(path, "w", encoding="utf-8") as f:
+            f.write(hdr + bssn_kernels(v, order))
+        paths.append(path)
+    for order in WAVE_BUILDS:
+        path = os.path.join(outdir, f"wave_fd{order}.hip")
+        with open(path, "w", encoding="utf-8") as f:
+            f.write(hdr + wave_kernels(order))
+        paths.append(path)
+    import json
+
+    with open(os.path.join(outdir, "kernel_meta.json"), "w", encoding="utf-8") as f:
+        json.dump({"flops_per_point": dict(sorted(FLOPS.items()))}, f, indent=1)
+    return paths
+
+
+if __name__ == "__main__":
+    here = os.path.dirname(os.path.abspath(__file__))
+    for p in generate(os.path.join(os.path.dirname(here), "csrc", "gen")):
+        print("wrote", p)

@@ -1,0 +1,938 @@
+// Host runtime and boundary-condition kernels of the gfx950 BHaH evolution backend.
+//
+// The generated per-variant translation units (gen/*.hip) hold the interior
+// stencil kernels; this file owns patch setup (grid, reference-metric
+// precompute arrays, boundary structs, CFL timestep), the boundary-condition
+// kernels, the RK4 method-of-lines driver and the C-ABI declared in
+// include/bhah_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/bhah_hip.h"
+
+struct bhah_kparams {
+  int Nx0, Nx1, Nx2, NG;
+  long Ntot, s1, s2;
+  double invdxx0, invdxx1, invdxx2;
+  double eta, KO_gauge, KO_nongauge, wavespeed;
+  double dt;
+  int substep;
+};
+
+// ---------------------------------------------------------------------------
+// Generated kernel launchers, one set per (coordinate, symmetry, KO, fd order).
+#define BHAH_DECL_BSSN(tag)                                                                                           \
+  extern "C" hipError_t launch_bssn_rhs_##tag(dim3, dim3, hipStream_t, const double *, const double *, double *,      \
+                                              const double *, const double *, bhah_kparams);                           \
+  extern "C" hipError_t launch_bssn_rk_det_##tag(dim3, dim3, hipStream_t, const double *, const double *, double *,   \
+                                                 double *, const double *, const double *, bhah_kparams);              \
+  extern "C" hipError_t launch_bssn_ricci_##tag(dim3, dim3, hipStream_t, const double *, double *, const double *,    \
+                                                const double *, bhah_kparams);                                         \
+  extern "C" hipError_t launch_bssn_detgamma_##tag(dim3, dim3, hipStream_t, double *, const double *, const double *, \
+                                                   bhah_kparams);                                                      \
+  extern "C" hipError_t launch_bssn_constraints_##tag(dim3, dim3, hipStream_t, const double *, const double *,        \
+                                                      double *, const double *, const double *, bhah_kparams);
+BHAH_DECL_BSSN(Spherical_sym2_KO0_fd4)
+BHAH_DECL_BSSN(SinhSpherical_symnone_KO1_fd6)
+BHAH_DECL_BSSN(SinhSpherical_symnone_KO1_fd4)
+extern "C" hipError_t launch_wave_rhs_fd4(dim3, dim3, hipStream_t, const double *, double *, bhah_kparams);
+extern "C" hipError_t launch_wave_rhs_rk_fd4(dim3, dim3, hipStream_t, const double *, const double *, double *,
+                                             double *, bhah_kparams);
+
+namespace {
+
+struct BssnKernels {
+  int coord, sym_axes, KO, fd_order;
+  hipError_t (*rhs)(dim3, dim3, hipStream_t, const double *, const double *, double *, const double *, const double *,
+                    bhah_kparams);
+  hipError_t (*rk_det)(dim3, dim3, hipStream_t, const double *, const double *, double *, double *, const double *,
+                       const double *, bhah_kparams);
+  hipError_t (*ricci)(dim3, dim3, hipStream_t, const double *, double *, const double *, const double *, bhah_kparams);
+  hipError_t (*detgamma)(dim3, dim3, hipStream_t, double *, const double *, const double *, bhah_kparams);
+  hipError_t (*constraints)(dim3, dim3, hipStream_t, const double *, const double *, double *, const double *,
+                            const double *, bhah_kparams);
+};
+
+#define BHAH_KSET(coord, sym, ko, fd, tag)                                                                            \
+  {coord, sym, ko, fd, launch_bssn_rhs_##tag, launch_bssn_rk_det_##tag, launch_bssn_ricci_##tag,                   \
+   launch_bssn_detgamma_##tag, launch_bssn_constraints_##tag}
+const BssnKernels kBssnKernels[] = {
+    BHAH_KSET(BHAH_SPHERICAL, 4, 0, 4, Spherical_sym2_KO0_fd4),
+    BHAH_KSET(BHAH_SINHSPHERICAL, 0, 1, 6, SinhSpherical_symnone_KO1_fd6),
+    BHAH_KSET(BHAH_SINHSPHERICAL, 0, 1, 4, SinhSpherical_symnone_KO1_fd4),
+};
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+#define HIP_TRY(call)                                                                                                 \
+  do {                                                                                                                \
+    hipError_t e_ = (call);                                                                                           \
+    if (e_ != hipSuccess) return fail(BHAH_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, \
+                                      __LINE__);                                                                      \
+  } while (0)
+
+// Parity type per evolved gridfunction (nrpy/grid.py:116-126, 270-321).
+const int kBssnParity[24] = {4, 5, 6, 7, 8, 9, 0, 1, 2, 3, 0, 4, 5, 6, 7, 8, 9, 1, 2, 3, 0, 1, 2, 3};
+
+}  // namespace
+
+struct bhah_patch {
+  bhah_patch_desc d;
+  const BssnKernels *k = nullptr;
+  int NG = 0, ngfs = 0, naux = 0;
+  int Nx[3] = {0, 0, 0};
+  long Ntot = 0;
+  double xxmin[3], xxmax[3], dxx[3], invdxx[3];
+  double dt = 0;
+  std::vector<double> xx[3];
+  // host boundary structs
+  std::vector<int64_t> inner_dst, inner_src;
+  std::vector<int8_t> inner_parity;  // 10 per point
+  std::vector<int16_t> outer[8][3];  // 6 shorts per point
+  // device arrays
+  double *d_rfm0 = nullptr, *d_rfm1 = nullptr, *d_xx0 = nullptr;
+  int *d_inner_dst = nullptr, *d_inner_src = nullptr;
+  unsigned short *d_inner_mask = nullptr;  // bit t set: parity type t flips sign
+  int16_t *d_outer[8][3] = {};
+  double *d_wavespeed = nullptr, *d_finf = nullptr;
+  double *d_red = nullptr;
+  int *d_gf_parity = nullptr;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  // cached graph for MoL_step_n
+  hipGraphExec_t graph_exec = nullptr;
+  bhah_MoL_gridfunctions graph_gf{};
+  double graph_dt = 0;
+
+  bhah_kparams kp(double dt_, int substep) const {
+    bhah_kparams p;
+    p.Nx0 = Nx[0];
+    p.Nx1 = Nx[1];
+    p.Nx2 = Nx[2];
+    p.NG = NG;
+    p.Ntot = Ntot;
+    p.s1 = Nx[0];
+    p.s2 = (long)Nx[0] * Nx[1];
+    p.invdxx0 = invdxx[0];
+    p.invdxx1 = invdxx[1];
+    p.invdxx2 = invdxx[2];
+    p.eta = d.eta;
+    p.KO_gauge = d.KO_strength_gauge;
+    p.KO_nongauge = d.KO_strength_nongauge;
+    p.wavespeed = d.wavespeed;
+    p.dt = dt_;
+    p.substep = substep;
+    return p;
+  }
+  // Interior launch: 64 x 4 threads over (i0, i1), one block row per i2.
+  void interior_dims(dim3 &g, dim3 &b) const {
+    b = dim3(64, 4, 1);
+    g = dim3((Nx[0] - 2 * NG + 63) / 64, (Nx[1] - 2 * NG + 3) / 4, Nx[2] - 2 * NG);
+  }
+  void all_dims(dim3 &g, dim3 &b) const {
+    b = dim3(64, 4, 1);
+    g = dim3((Nx[0] + 63) / 64, (Nx[1] + 3) / 4, Nx[2]);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Boundary-condition kernels.
+
+// Sommerfeld outgoing-radiation RHS at pure outer points of one (layer, direction)
+// list, spherical-like coordinates (CurviBoundaryConditions/tests/
+// apply_bcs_outerradiation_and_inner__rfm__openmp__HoleySinhSpherical.c:42-198).
+// One thread per (point, gridfunction).
+struct RadStencil {
+  int n[7];
+  int off[7][7];
+  double w[7][7];
+};
+
+__device__ __forceinline__ void r_and_dx0dr(int coord, double AMPL, double SINHW, double xx0, double &r, double &dx0dr) {
+#pragma clang fp contract(off)
+  if (coord == BHAH_SPHERICAL) {
+    r = xx0;
+    dx0dr = 1.0;
+  } else {
+    const double tmp0 = (1.0 / (SINHW));
+    const double tmp1 = exp(tmp0) - exp(-tmp0);
+    const double tmp3 = exp(tmp0 * xx0);
+    const double tmp4 = exp(-tmp0 * xx0);
+    r = AMPL * (tmp3 - tmp4) / tmp1;
+    dx0dr = tmp1 / (AMPL * (tmp0 * tmp3 + tmp0 * tmp4));
+  }
+}
+
+__device__ __forceinline__ double partial_r_f(const RadStencil &st, int R, const double *__restrict__ f, int Nx0,
+                                              long idx, int i0, int FACE0, double dx0dr, double invdxx0) {
+#pragma clang fp contract(off)
+  int off = FACE0;
+  if (i0 < R)
+    off = R - i0;
+  else if (i0 > Nx0 - R - 1)
+    off = (Nx0 - R - 1) - i0;
+  const int k = off + R;
+  double acc = 0.0;
+  for (int j = 0; j < st.n[k]; j++) acc += st.w[k][j] * f[idx + st.off[k][j]];
+  return dx0dr * (acc * invdxx0);
+}
+
+__global__ void __launch_bounds__(256)
+bc_radiation_kernel(const int16_t *__restrict__ pts, int npts, int ngfs, const double *__restrict__ gfs,
+                    double *__restrict__ rhs, const double *__restrict__ xx0, const double *__restrict__ wavespeed,
+                    const double *__restrict__ finf, RadStencil st, int R, int coord, double AMPL, double SINHW,
+                    int Nx0, long s1, long s2, long Ntot, double invdxx0) {
+#pragma clang fp contract(off)
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)npts * ngfs) return;
+  const int p = (int)(t % npts), gf = (int)(t / npts);
+  const int16_t *q = &pts[6 * p];
+  const int i0 = q[0], i1 = q[1], i2 = q[2];
+  const int j0 = i0 + q[3], j1 = i1 + q[4], j2 = i2 + q[5];
+  double r, dxr, r_int, dxr_int;
+  r_and_dx0dr(coord, AMPL, SINHW, xx0[i0], r, dxr);
+  r_and_dx0dr(coord, AMPL, SINHW, xx0[j0], r_int, dxr_int);
+  const long idx = i0 + s1 * i1 + s2 * i2;
+  const long idx_int = j0 + s1 * j1 + s2 * j2;
+  const double *f = &gfs[gf * Ntot];
+  const double dfr = partial_r_f(st, R, f, Nx0, idx, i0, q[3], dxr, invdxx0);
+  const double dfr_int = partial_r_f(st, R, f, Nx0, idx_int, j0, q[3], dxr_int, invdxx0);
+  const double c = wavespeed[gf], f_inf = finf[gf];
+  const double dtf_int = rhs[gf * Ntot + idx_int];
+  const double dtf_int_out = -c * (dfr_int + (f[idx_int] - f_inf) / r_int);
+  const double k = r_int * r_int * r_int * (dtf_int - dtf_int_out);
+  const double rinv = 1.0 / r;
+  const double dtf_out = -c * (dfr + (f[idx] - f_inf) * rinv);
+  rhs[gf * Ntot + idx] = dtf_out + k * rinv * rinv * rinv;
+}
+
+// Quadratic extrapolation f0 = 3 f1 - 3 f2 + f3 along the face normal
+// (…/apply_bcs_outerextrap_and_inner__openmp.c:8-34).
+__global__ void __launch_bounds__(256)
+bc_extrap_kernel(const int16_t *__restrict__ pts, int npts, int ngfs, double *__restrict__ gfs, long s1, long s2,
+                 long Ntot) {
+#pragma clang fp contract(off)
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)npts * ngfs) return;
+  const int p = (int)(t % npts), gf = (int)(t / npts);
+  const int16_t *q = &pts[6 * p];
+  const long i = q[0] + s1 * q[1] + s2 * q[2];
+  const long s = q[3] + s1 * q[4] + s2 * q[5];
+  double *f = &gfs[gf * Ntot];
+  f[i] = +3.0 * f[i + s] - 3.0 * f[i + 2 * s] + 1.0 * f[i + 3 * s];
+}
+
+// gf[dst] = +-gf[src] by a sign-bit flip (…/apply_bcs_inner_only__openmp.c:3-57).
+__global__ void __launch_bounds__(256)
+bc_inner_kernel(const int *__restrict__ dst, const int *__restrict__ src, const unsigned short *__restrict__ mask,
+                int npts, int ngfs, const int *__restrict__ gf_parity, double *__restrict__ gfs, long Ntot) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)npts * ngfs) return;
+  const int p = (int)(t % npts), gf = (int)(t / npts);
+  double *f = &gfs[gf * Ntot];
+  const double v = f[src[p]];
+  f[dst[p]] = ((mask[p] >> gf_parity[gf]) & 1u) ? -v : v;
+}
+
+// Volume-weighted sums over the interior for the constraint norms.
+__global__ void __launch_bounds__(256)
+l2_reduce_kernel(const double *__restrict__ h, const double *__restrict__ msq, const double *__restrict__ rfm0,
+                 const double *__restrict__ rfm1, bhah_kparams p, double dvol, double *__restrict__ out) {
+  __shared__ double sh[3][256];
+  const int i0 = p.NG + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int i1 = p.NG + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const int i2 = p.NG + (int)blockIdx.z;
+  double a = 0, b = 0, c = 0;
+  if (i0 < p.Nx0 - p.NG && i1 < p.Nx1 - p.NG) {
+    const long idx = i0 + p.s1 * i1 + p.s2 * i2;
+    const double f0 = rfm0[i0], f0p = rfm0[p.Nx0 + i0], f1 = rfm1[i1];
+    const double dV = fabs(f0p * f0 * f0 * f1) * dvol;
+    a = dV;
+    b = h[idx] * h[idx] * dV;
+    c = msq[idx] * dV;
+  }
+  const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+  sh[0][tid] = a;
+  sh[1][tid] = b;
+  sh[2][tid] = c;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (tid < s) {
+      sh[0][tid] += sh[0][tid + s];
+      sh[1][tid] += sh[1][tid + s];
+      sh[2][tid] += sh[2][tid + s];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    atomicAdd(&out[0], sh[0][0]);
+    atomicAdd(&out[1], sh[1][0]);
+    atomicAdd(&out[2], sh[2][0]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host-side setup.
+namespace {
+
+// First-derivative weights on 2R+1 points shifted by o (exact rationals, then rounded once):
+// Vandermonde solve, as nrpy/finite_difference.py:93-131 with the radiation BC's
+// arbitrary-offset stencils (CurviBoundaryConditions/apply_bcs_outerradiation_and_inner.py:126-160).
+void rad_stencil(int R, RadStencil &st) {
+  memset(&st, 0, sizeof(st));
+  for (int o = -R; o <= R; o++) {
+    const int n = 2 * R + 1;
+    // solve sum_j w_j x_j^i = delta_{i1} with x_j = j - R + o, by Lagrange basis derivative at 0
+    int cnt = 0;
+    for (int j = 0; j < n; j++) {
+      const int xj = j - R + o;
+      // w_j = L_j'(0) = sum_{m != j} [1/(xj - xm)] prod_{l != j, m} (0 - xl)/(xj - xl)
+      long double w = 0.0L;
+      for (int m = 0; m < n; m++) {
+        if (m == j) continue;
+        const int xm = m - R + o;
+        long double term = 1.0L / (long double)(xj - xm);
+        for (int l = 0; l < n; l++) {
+          if (l == j || l == m) continue;
+          const int xl = l - R + o;
+          term *= (long double)(0 - xl) / (long double)(xj - xl);
+        }
+        w += term;
+      }
+      if (w != 0.0L) {
+        st.off[o + R][cnt] = xj;
+        st.w[o + R][cnt] = (double)w;
+        cnt++;
+      }
+    }
+    st.n[o + R] = cnt;
+  }
+}
+
+// Spherical eigen-coordinate double map xx -> Cart -> xx' (bcstruct_set_up…HoleySinhSpherical.c:43-185).
+void eigen_inbounds(const bhah_patch &P, int i0, int i1, int i2, int ib[3], double xin[3]) {
+  if (P.d.coord == BHAH_CARTESIAN) {
+    ib[0] = i0;
+    ib[1] = i1;
+    ib[2] = i2;
+    for (int d = 0; d < 3; d++) xin[d] = P.xx[d][ib[d]];
+    return;
+  }
+  const double r = P.xx[0][i0], th = P.xx[1][i1], ph = P.xx[2][i2];
+  const double X = r * sin(th) * cos(ph), Y = r * sin(th) * sin(ph), Z = r * cos(th);
+  const double rr = sqrt(X * X + Y * Y + Z * Z);
+  const double c[3] = {rr, acos(Z / rr), atan2(Y, X)};
+  for (int d = 0; d < 3; d++) {
+    const double xxmin = P.xx[d][0] - ((double)(0 - P.NG) + (1.0 / 2.0)) * P.dxx[d];
+    ib[d] = (int)((c[d] - xxmin - (1.0 / 2.0) * P.dxx[d] + ((double)P.NG) * P.dxx[d]) / P.dxx[d] + 0.5);
+  }
+  for (int d = 0; d < 3; d++) xin[d] = P.xx[d][ib[d]];
+}
+
+int parity_signs(double xx1, double xx2, const double xin[3], int8_t par[10]) {
+  const double t0 = cos(xx1) * cos(xin[1]);
+  const double t1 = sin(xx1) * sin(xin[1]);
+  const double t2 = sin(xx2) * sin(xin[2]);
+  const double t3 = cos(xx2) * cos(xin[2]);
+  const double p1 = t0 + t1 * t2 + t1 * t3;
+  const double p2 = t0 * t2 + t0 * t3 + t1;
+  const double p3 = t2 + t3;
+  const double dots[10] = {1, p1, p2, p3, p1 * p1, p1 * p2, p1 * p3, p2 * p2, p2 * p3, p3 * p3};
+  for (int k = 0; k < 10; k++) {
+    if (fabs(dots[k]) < 1 - 1e-8 || fabs(dots[k]) > 1 + 1e-8) return -1;
+    par[k] = dots[k] < 0 ? -1 : 1;
+  }
+  return 0;
+}
+
+int build_bcstruct(bhah_patch &P) {
+  const int NG = P.NG;
+  const int *Nx = P.Nx;
+  for (int i2 = 0; i2 < Nx[2]; i2++)
+    for (int i1 = 0; i1 < Nx[1]; i1++)
+      for (int i0 = 0; i0 < Nx[0]; i0++) {
+        const bool interior =
+            i0 >= NG && i0 < Nx[0] - NG && i1 >= NG && i1 < Nx[1] - NG && i2 >= NG && i2 < Nx[2] - NG;
+        if (interior) continue;
+        int ib[3];
+        double xin[3];
+        eigen_inbounds(P, i0, i1, i2, ib, xin);
+        if (ib[0] == i0 && ib[1] == i1 && ib[2] == i2) continue;
+        for (int d = 0; d < 3; d++)
+          if (ib[d] < 0 || ib[d] >= Nx[d])
+            return fail(BHAH_ERR_BCSTRUCT, "inner boundary point (%d,%d,%d) maps outside the grid", i0, i1, i2);
+        int8_t par[10];
+        if (parity_signs(P.xx[1][i1], P.xx[2][i2], xin, par))
+          return fail(BHAH_ERR_BCSTRUCT, "parity dot product not +-1 at (%d,%d,%d)", i0, i1, i2);
+        P.inner_dst.push_back(i0 + (int64_t)Nx[0] * (i1 + (int64_t)Nx[1] * i2));
+        P.inner_src.push_back(ib[0] + (int64_t)Nx[0] * (ib[1] + (int64_t)Nx[1] * ib[2]));
+        for (int k = 0; k < 10; k++) P.inner_parity.push_back(par[k]);
+      }
+  // outer boundary lists, innermost layer first; faces x0min,x0max,x1min,... each using the
+  // current bounds and then widening its own direction (bcstruct_set_up…HoleySinhSpherical.c:382-520)
+  int imin[3] = {NG, NG, NG};
+  int imax[3] = {Nx[0] - NG, Nx[1] - NG, Nx[2] - NG};
+  for (int gz = 0; gz < NG; gz++)
+    for (int d = 0; d < 3; d++)
+      for (int side = 0; side < 2; side++) {
+        int r[6];
+        for (int e = 0; e < 3; e++) {
+          r[2 * e] = imin[e];
+          r[2 * e + 1] = imax[e];
+        }
+        if (side == 0) {
+          r[2 * d] = imin[d] - 1;
+          r[2 * d + 1] = imin[d];
+          imin[d]--;
+        } else {
+          r[2 * d] = imax[d];
+          r[2 * d + 1] = imax[d] + 1;
+          imax[d]++;
+        }
+        const int face = 2 * d + side;
+        const int F0 = (face == 0) - (face == 1), F1 = (face == 2) - (face == 3), F2 = (face == 4) - (face == 5);
+        for (int i2 = r[4]; i2 < r[5]; i2++)
+          for (int i1 = r[2]; i1 < r[3]; i1++)
+            for (int i0 = r[0]; i0 < r[1]; i0++) {
+              int ib[3];
+              double xin[3];
+              eigen_inbounds(P, i0, i1, i2, ib, xin);
+              if (ib[0] == i0 && ib[1] == i1 && ib[2] == i2) {
+                const int16_t v[6] = {(int16_t)i0, (int16_t)i1, (int16_t)i2, (int16_t)F0, (int16_t)F1, (int16_t)F2};
+                P.outer[gz][d].insert(P.outer[gz][d].end(), v, v + 6);
+              }
+            }
+      }
+  return BHAH_OK;
+}
+
+template <typename T>
+int upload(const std::vector<T> &h, T **dptr) {
+  *dptr = nullptr;
+  if (h.empty()) return BHAH_OK;
+  HIP_TRY(hipMalloc((void **)dptr, sizeof(T) * h.size()));
+  HIP_TRY(hipMemcpy(*dptr, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+  return BHAH_OK;
+}
+
+int launch_1d(long n, dim3 &g, dim3 &b) {
+  b = dim3(256);
+  g = dim3((unsigned)((n + 255) / 256));
+  return BHAH_OK;
+}
+
+int check_ptr(const void *p, const char *name) {
+  if (!p) return fail(BHAH_ERR_ARG, "%s is NULL", name);
+  return BHAH_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C-ABI.
+extern "C" {
+
+const char *bhah_hip_last_error(void) { return g_last_error.c_str(); }
+int bhah_hip_abi_version(void) { return BHAH_HIP_ABI_VERSION; }
+
+int bhah_hip_patch_create(const bhah_patch_desc *desc, bhah_patch **out) {
+  if (!desc || !out) return fail(BHAH_ERR_ARG, "NULL argument");
+  *out = nullptr;
+  const bhah_patch_desc &d = *desc;
+  if (d.fd_order != 2 && d.fd_order != 4 && d.fd_order != 6 && d.fd_order != 8)
+    return fail(BHAH_ERR_ARG, "fd_order %d unsupported", d.fd_order);
+  for (int i = 0; i < 3; i++)
+    if (d.Nxx[i] < 1) return fail(BHAH_ERR_ARG, "Nxx[%d] = %d", i, d.Nxx[i]);
+  bhah_patch *P = new bhah_patch();
+  P->d = d;
+  if (d.system == BHAH_SYSTEM_BSSN) {
+    for (const auto &k : kBssnKernels)
+      if (k.coord == d.coord && k.sym_axes == d.symmetry_axes && k.KO == (d.enable_KO != 0) && k.fd_order == d.fd_order)
+        P->k = &k;
+    if (!P->k) {
+      delete P;
+      return fail(BHAH_ERR_UNSUPPORTED, "no BSSN kernel build for coord=%d symmetry_axes=%d KO=%d fd_order=%d", d.coord,
+                  d.symmetry_axes, d.enable_KO, d.fd_order);
+    }
+    P->NG = d.fd_order / 2 + 1;  // upwinding / KO need one extra ghost zone (BHaH_defines_h.py:278-292)
+    P->ngfs = BHAH_NUM_EVOL_GFS_BSSN;
+    P->naux = BHAH_NUM_AUXEVOL_GFS_BSSN;
+  } else if (d.system == BHAH_SYSTEM_WAVE) {
+    if (d.coord != BHAH_CARTESIAN || d.fd_order != 4) {
+      delete P;
+      return fail(BHAH_ERR_UNSUPPORTED, "wave system supports Cartesian, fd_order 4 only");
+    }
+    P->NG = d.fd_order / 2;
+    P->ngfs = BHAH_NUM_EVOL_GFS_WAVE;
+  } else {
+    delete P;
+    return fail(BHAH_ERR_ARG, "unknown system %d", d.system);
+  }
+  // Grid (numerical_grid_params_Nxx_dxx_xx…HoleySinhSpherical.c:60-104; reference_metric.py:1201-1274).
+  if (d.coord == BHAH_CARTESIAN) {
+    for (int i = 0; i < 3; i++) {
+      P->xxmin[i] = -d.grid_physical_size;
+      P->xxmax[i] = d.grid_physical_size;
+    }
+  } else {
+    P->xxmin[0] = 0.0;
+    P->xxmax[0] = (d.coord == BHAH_SPHERICAL) ? d.grid_physical_size : 1.0;
+    P->xxmin[1] = 0.0;
+    P->xxmax[1] = M_PI;
+    P->xxmin[2] = -M_PI;
+    P->xxmax[2] = M_PI;
+  }
+  for (int i = 0; i < 3; i++) {
+    P->Nx[i] = d.Nxx[i] + 2 * P->NG;
+    P->dxx[i] = (P->xxmax[i] - P->xxmin[i]) / ((double)d.Nxx[i]);
+    P->invdxx[i] = ((double)d.Nxx[i]) / (P->xxmax[i] - P->xxmin[i]);
+    P->xx[i].resize(P->Nx[i]);
+    for (int j = 0; j < P->Nx[i]; j++) P->xx[i][j] = P->xxmin[i] + ((double)(j - P->NG) + (1.0 / 2.0)) * P->dxx[i];
+  }
+  P->Ntot = (long)P->Nx[0] * P->Nx[1] * P->Nx[2];
+  if (P->Nx[0] > 32767 || P->Nx[1] > 32767 || P->Nx[2] > 32767 || P->Ntot * (long)P->ngfs > (1L << 31) * 64) {
+    delete P;
+    return fail(BHAH_ERR_ARG, "grid too large");
+  }
+  // rfm precompute arrays: f0, f0', f0'', f0''' over xx0 and f1, f1', f1'' over xx1
+  // (rfm_precompute.py:196-340 with the closed forms of reference_metric.py:1338-1357).
+  std::vector<double> rfm0(4 * P->Nx[0]), rfm1(3 * P->Nx[1]);
+  for (int i = 0; i < P->Nx[0]; i++) {
+    const double x = P->xx[0][i];
+    double f = 1, d1 = 0, d2 = 0, d3 = 0;
+    if (d.coord == BHAH_SPHERICAL) {
+      f = x;
+      d1 = 1.0;
+    } else if (d.coord == BHAH_SINHSPHERICAL) {
+      const double A = d.grid_physical_size, w = d.SINHW;
+      const double den = exp(1.0 / w) - exp(-1.0 / w);
+      const double ep = exp(x / w), em = exp(-x / w);
+      f = A * (ep - em) / den;
+      d1 = A * (ep + em) / (w * den);
+      d2 = A * (ep - em) / (w * w * den);
+      d3 = A * (ep + em) / (w * w * w * den);
+    }
+    rfm0[i] = f;
+    rfm0[P->Nx[0] + i] = d1;
+    rfm0[2 * P->Nx[0] + i] = d2;
+    rfm0[3 * P->Nx[0] + i] = d3;
+  }
+  for (int j = 0; j < P->Nx[1]; j++) {
+    const double y = P->xx[1][j];
+    const bool cart = d.coord == BHAH_CARTESIAN;
+    rfm1[j] = cart ? 1.0 : sin(y);
+    rfm1[P->Nx[1] + j] = cart ? 0.0 : cos(y);
+    rfm1[2 * P->Nx[1] + j] = cart ? 0.0 : -sin(y);
+  }
+  int rc = build_bcstruct(*P);
+  if (rc) {
+    delete P;
+    return rc;
+  }
+  // CFL timestep: dt = CFL * min over all points of min_i ds_i
+  // (numerical_grids_and_timestep_cfl_limited_timestep__openmp.c; ds_min_single_pt…HoleySinhSpherical.c).
+  double ds_min = 1e38;
+  if (d.coord == BHAH_CARTESIAN) {
+    ds_min = fmin(P->dxx[0], fmin(P->dxx[1], P->dxx[2]));
+  } else {
+    for (int i2 = 0; i2 < P->Nx[2]; i2++)
+      for (int i1 = 0; i1 < P->Nx[1]; i1++)
+        for (int i0 = 0; i0 < P->Nx[0]; i0++) {
+          const double x0 = P->xx[0][i0], x1 = P->xx[1][i1];
+          double ds0, ds1, ds2;
+          if (d.coord == BHAH_SPHERICAL) {
+            ds0 = fabs(P->dxx[0]);
+            ds1 = fabs(x0 * P->dxx[1]);
+            ds2 = fabs(x0 * P->dxx[2] * sin(x1));
+          } else {
+            const double tmp0 = (1.0 / (d.SINHW));
+            const double tmp4 = d.grid_physical_size / (exp(tmp0) - exp(-tmp0));
+            const double tmp2 = exp(tmp0 * x0);
+            const double tmp3 = exp(-tmp0 * x0);
+            const double tmp5 = tmp4 * (tmp2 - tmp3);
+            ds0 = fabs(P->dxx[0] * tmp4 * (tmp0 * tmp2 + tmp0 * tmp3));
+            ds1 = fabs(P->dxx[1] * tmp5);
+            ds2 = fabs(P->dxx[2] * tmp5 * sin(x1));
+          }
+          const double m = fmin(ds0, fmin(ds1, ds2));
+          ds_min = m < ds_min ? m : ds_min;
+        }
+  }
+  P->dt = d.CFL_FACTOR * ds_min;
+  // device data
+  std::vector<unsigned short> mask(P->inner_dst.size());
+  std::vector<int> dst(P->inner_dst.size()), src(P->inner_dst.size());
+  for (size_t p = 0; p < mask.size(); p++) {
+    unsigned m = 0;
+    for (int t = 0; t < 10; t++)
+      if (P->inner_parity[10 * p + t] < 0) m |= 1u << t;
+    mask[p] = (unsigned short)m;
+    dst[p] = (int)P->inner_dst[p];
+    src[p] = (int)P->inner_src[p];
+  }
+  std::vector<double> ws(P->ngfs, 1.0), fi(P->ngfs, 0.0);
+  if (d.system == BHAH_SYSTEM_BSSN) {
+    ws[6] = sqrt(2.0);  // alpha (BSSN_quantities.py:62-86)
+    fi[6] = 1.0;        // alpha
+    fi[10] = 1.0;       // cf
+  }
+  std::vector<int> par(kBssnParity, kBssnParity + 24);
+  if ((rc = upload(rfm0, &P->d_rfm0)) || (rc = upload(rfm1, &P->d_rfm1)) || (rc = upload(P->xx[0], &P->d_xx0)) ||
+      (rc = upload(dst, &P->d_inner_dst)) || (rc = upload(src, &P->d_inner_src)) ||
+      (rc = upload(mask, &P->d_inner_mask)) || (rc = upload(ws, &P->d_wavespeed)) || (rc = upload(fi, &P->d_finf)) ||
+      (rc = upload(par, &P->d_gf_parity))) {
+    bhah_hip_patch_destroy(P);
+    return rc;
+  }
+  for (int gz = 0; gz < P->NG; gz++)
+    for (int dd = 0; dd < 3; dd++)
+      if ((rc = upload(P->outer[gz][dd], &P->d_outer[gz][dd]))) {
+        bhah_hip_patch_destroy(P);
+        return rc;
+      }
+  if (hipMalloc((void **)&P->d_red, 8 * sizeof(double)) != hipSuccess) {
+    bhah_hip_patch_destroy(P);
+    return fail(BHAH_ERR_HIP, "hipMalloc failed");
+  }
+  P->stream = nullptr;  // default (null) stream unless set
+  *out = P;
+  return BHAH_OK;
+}
+
+int bhah_hip_patch_destroy(bhah_patch *P) {
+  if (!P) return BHAH_OK;
+  if (P->graph_exec) hipGraphExecDestroy(P->graph_exec);
+  hipFree(P->d_rfm0);
+  hipFree(P->d_rfm1);
+  hipFree(P->d_xx0);
+  hipFree(P->d_inner_dst);
+  hipFree(P->d_inner_src);
+  hipFree(P->d_inner_mask);
+  hipFree(P->d_wavespeed);
+  hipFree(P->d_finf);
+  hipFree(P->d_red);
+  hipFree(P->d_gf_parity);
+  for (int gz = 0; gz < 8; gz++)
+    for (int d = 0; d < 3; d++) hipFree(P->d_outer[gz][d]);
+  if (P->own_stream) hipStreamDestroy(P->stream);
+  delete P;
+  return BHAH_OK;
+}
+
+int bhah_hip_patch_get_info(const bhah_patch *P, bhah_patch_info *info) {
+  if (!P || !info) return fail(BHAH_ERR_ARG, "NULL argument");
+  memset(info, 0, sizeof(*info));
+  for (int i = 0; i < 3; i++) {
+    info->Nxx[i] = P->d.Nxx[i];
+    info->Nxx_plus_2NGHOSTS[i] = P->Nx[i];
+    info->xxmin[i] = P->xxmin[i];
+    info->xxmax[i] = P->xxmax[i];
+    info->dxx[i] = P->dxx[i];
+    info->invdxx[i] = P->invdxx[i];
+  }
+  info->NGHOSTS = P->NG;
+  info->num_evol_gfs = P->ngfs;
+  info->num_auxevol_gfs = P->naux;
+  info->Ntot = P->Ntot;
+  info->dt = P->dt;
+  info->num_inner_boundary_points = (int)P->inner_dst.size();
+  int n = 0;
+  for (int gz = 0; gz < P->NG; gz++)
+    for (int d = 0; d < 3; d++) n += (int)(P->outer[gz][d].size() / 6);
+  info->num_pure_outer_boundary_points = n;
+  return BHAH_OK;
+}
+
+int bhah_hip_patch_set_stream(bhah_patch *P, void *s) {
+  if (!P) return fail(BHAH_ERR_ARG, "NULL patch");
+  P->stream = (hipStream_t)s;
+  if (P->graph_exec) {
+    hipGraphExecDestroy(P->graph_exec);
+    P->graph_exec = nullptr;
+  }
+  return BHAH_OK;
+}
+
+int bhah_hip_synchronize(bhah_patch *P) {
+  if (!P) return fail(BHAH_ERR_ARG, "NULL patch");
+  HIP_TRY(hipStreamSynchronize(P->stream));
+  return BHAH_OK;
+}
+
+int bhah_hip_patch_get_xx(const bhah_patch *P, double *x0, double *x1, double *x2) {
+  if (!P || !x0 || !x1 || !x2) return fail(BHAH_ERR_ARG, "NULL argument");
+  double *o[3] = {x0, x1, x2};
+  for (int d = 0; d < 3; d++) memcpy(o[d], P->xx[d].data(), sizeof(double) * P->Nx[d]);
+  return BHAH_OK;
+}
+
+int bhah_hip_patch_get_inner_bc(const bhah_patch *P, int64_t *dst, int64_t *src, int8_t *par) {
+  if (!P || !dst || !src || !par) return fail(BHAH_ERR_ARG, "NULL argument");
+  memcpy(dst, P->inner_dst.data(), sizeof(int64_t) * P->inner_dst.size());
+  memcpy(src, P->inner_src.data(), sizeof(int64_t) * P->inner_src.size());
+  memcpy(par, P->inner_parity.data(), P->inner_parity.size());
+  return BHAH_OK;
+}
+
+int bhah_hip_patch_get_outer_bc_count(const bhah_patch *P, int gz, int d) {
+  if (!P || gz < 0 || gz >= P->NG || d < 0 || d > 2) return fail(BHAH_ERR_ARG, "bad arguments");
+  return (int)(P->outer[gz][d].size() / 6);
+}
+
+int bhah_hip_patch_get_outer_bc(const bhah_patch *P, int gz, int d, int16_t *out) {
+  if (!P || !out || gz < 0 || gz >= P->NG || d < 0 || d > 2) return fail(BHAH_ERR_ARG, "bad arguments");
+  memcpy(out, P->outer[gz][d].data(), sizeof(int16_t) * P->outer[gz][d].size());
+  return BHAH_OK;
+}
+
+// ---- interior kernels
+int bhah_hip_rhs_eval(bhah_patch *P, const double *aux, const double *in, double *rhs) {
+  if (!P || !P->k) return fail(BHAH_ERR_ARG, "patch is not a BSSN patch");
+  int rc;
+  if ((rc = check_ptr(aux, "auxevol_gfs")) || (rc = check_ptr(in, "in_gfs")) || (rc = check_ptr(rhs, "rhs_gfs"))) return rc;
+  dim3 g, b;
+  P->interior_dims(g, b);
+  HIP_TRY(P->k->rhs(g, b, P->stream, in, aux, rhs, P->d_rfm0, P->d_rfm1, P->kp(0, 0)));
+  return BHAH_OK;
+}
+
+int bhah_hip_Ricci_eval(bhah_patch *P, const double *in, double *aux) {
+  if (!P || !P->k) return fail(BHAH_ERR_ARG, "patch is not a BSSN patch");
+  int rc;
+  if ((rc = check_ptr(in, "in_gfs")) || (rc = check_ptr(aux, "auxevol_gfs"))) return rc;
+  dim3 g, b;
+  P->interior_dims(g, b);
+  HIP_TRY(P->k->ricci(g, b, P->stream, in, aux, P->d_rfm0, P->d_rfm1, P->kp(0, 0)));
+  return BHAH_OK;
+}
+
+int bhah_hip_enforce_detgammabar_equals_detgammahat(bhah_patch *P, double *gfs) {
+  if (!P || !P->k) return fail(BHAH_ERR_ARG, "patch is not a BSSN patch");
+  if (!gfs) return fail(BHAH_ERR_ARG, "in_gfs is NULL");
+  dim3 g, b;
+  P->all_dims(g, b);
+  HIP_TRY(P->k->detgamma(g, b, P->stream, gfs, P->d_rfm0, P->d_rfm1, P->kp(0, 0)));
+  return BHAH_OK;
+}
+
+int bhah_hip_apply_bcs_inner_only(bhah_patch *P, double *gfs) {
+  if (!P || !gfs) return fail(BHAH_ERR_ARG, "NULL argument");
+  const int n = (int)P->inner_dst.size();
+  if (n == 0) return BHAH_OK;
+  dim3 g, b;
+  launch_1d((long)n * P->ngfs, g, b);
+  hipLaunchKernelGGL(bc_inner_kernel, g, b, 0, P->stream, P->d_inner_dst, P->d_inner_src, P->d_inner_mask, n, P->ngfs,
+                     P->d_gf_parity, gfs, P->Ntot);
+  HIP_TRY(hipGetLastError());
+  return BHAH_OK;
+}
+
+int bhah_hip_apply_bcs_outerextrap_and_inner(bhah_patch *P, double *gfs) {
+  if (!P || !gfs) return fail(BHAH_ERR_ARG, "NULL argument");
+  for (int gz = 0; gz < P->NG; gz++)
+    for (int d = 0; d < 3; d++) {
+      const int n = (int)(P->outer[gz][d].size() / 6);
+      if (!n) continue;
+      dim3 g, b;
+      launch_1d((long)n * P->ngfs, g, b);
+      hipLaunchKernelGGL(bc_extrap_kernel, g, b, 0, P->stream, P->d_outer[gz][d], n, P->ngfs, gfs, (long)P->Nx[0],
+                         (long)P->Nx[0] * P->Nx[1], P->Ntot);
+      HIP_TRY(hipGetLastError());
+    }
+  return bhah_hip_apply_bcs_inner_only(P, gfs);
+}
+
+int bhah_hip_apply_bcs_outerradiation_and_inner(bhah_patch *P, const double *gfs, double *rhs) {
+  if (!P || !gfs || !rhs) return fail(BHAH_ERR_ARG, "NULL argument");
+  if (P->d.coord == BHAH_CARTESIAN) return fail(BHAH_ERR_UNSUPPORTED, "radiation BCs implemented for spherical-like coordinates");
+  const int R = (P->d.radiation_bc_fd_order > 0 ? P->d.radiation_bc_fd_order : 4) / 2;
+  if (R < 1 || R > 3) return fail(BHAH_ERR_ARG, "radiation_bc_fd_order must be 2, 4 or 6");
+  RadStencil st;
+  rad_stencil(R, st);
+  for (int gz = 0; gz < P->NG; gz++)
+    for (int d = 0; d < 3; d++) {
+      const int n = (int)(P->outer[gz][d].size() / 6);
+      if (!n) continue;
+      dim3 g, b;
+      launch_1d((long)n * P->ngfs, g, b);
+      hipLaunchKernelGGL(bc_radiation_kernel, g, b, 0, P->stream, P->d_outer[gz][d], n, P->ngfs, gfs, rhs, P->d_xx0,
+                         P->d_wavespeed, P->d_finf, st, R, P->d.coord, P->d.grid_physical_size, P->d.SINHW, P->Nx[0],
+                         (long)P->Nx[0], (long)P->Nx[0] * P->Nx[1], P->Ntot, P->invdxx[0]);
+      HIP_TRY(hipGetLastError());
+    }
+  return bhah_hip_apply_bcs_inner_only(P, rhs);
+}
+
+int bhah_hip_MoL_malloc_intermediate_stage_gfs(bhah_patch *P, bhah_MoL_gridfunctions *gf) {
+  if (!P || !gf) return fail(BHAH_ERR_ARG, "NULL argument");
+  const size_t bytes = sizeof(double) * (size_t)P->Ntot * P->ngfs;
+  if (!gf->y_n_gfs) HIP_TRY(hipMalloc((void **)&gf->y_n_gfs, bytes));
+  HIP_TRY(hipMalloc((void **)&gf->y_nplus1_running_total_gfs, bytes));
+  HIP_TRY(hipMalloc((void **)&gf->k_odd_gfs, bytes));
+  HIP_TRY(hipMalloc((void **)&gf->k_even_gfs, bytes));
+  if (P->naux) HIP_TRY(hipMalloc((void **)&gf->auxevol_gfs, sizeof(double) * (size_t)P->Ntot * P->naux));
+  const size_t all = bytes;
+  HIP_TRY(hipMemsetAsync(gf->y_nplus1_running_total_gfs, 0, all, P->stream));
+  HIP_TRY(hipMemsetAsync(gf->k_odd_gfs, 0, all, P->stream));
+  HIP_TRY(hipMemsetAsync(gf->k_even_gfs, 0, all, P->stream));
+  if (P->naux) HIP_TRY(hipMemsetAsync(gf->auxevol_gfs, 0, sizeof(double) * (size_t)P->Ntot * P->naux, P->stream));
+  return BHAH_OK;
+}
+
+int bhah_hip_MoL_free(bhah_patch *P, bhah_MoL_gridfunctions *gf) {
+  if (!P || !gf) return fail(BHAH_ERR_ARG, "NULL argument");
+  hipFree(gf->y_nplus1_running_total_gfs);
+  hipFree(gf->k_odd_gfs);
+  hipFree(gf->k_even_gfs);
+  hipFree(gf->auxevol_gfs);
+  gf->y_nplus1_running_total_gfs = gf->k_odd_gfs = gf->k_even_gfs = gf->auxevol_gfs = nullptr;
+  return BHAH_OK;
+}
+
+// Enqueue the four RK4 substeps (no host synchronisation).
+static int enqueue_step(bhah_patch *P, bhah_MoL_gridfunctions *gf, double dt) {
+  double *in_s[4] = {gf->y_n_gfs, gf->k_odd_gfs, gf->k_even_gfs, gf->k_odd_gfs};
+  double *out_s[4] = {gf->k_odd_gfs, gf->k_even_gfs, gf->k_odd_gfs, gf->k_even_gfs};
+  int rc;
+  if (P->d.system == BHAH_SYSTEM_WAVE) {
+    // rhs_eval + RK update fused at interior points, then extrapolation BCs on the new stage
+    // (nrpy/examples/wave_equation_cartesian.py:440-444).
+    for (int s = 1; s <= 4; s++) {
+      dim3 g, b;
+      P->interior_dims(g, b);
+      double *next = (s == 4) ? gf->y_n_gfs : out_s[s - 1];
+      HIP_TRY(launch_wave_rhs_rk_fd4(g, b, P->stream, in_s[s - 1], gf->y_n_gfs, gf->y_nplus1_running_total_gfs, next,
+                                     P->kp(dt, s)));
+      if ((rc = bhah_hip_wave_apply_bcs(P, next))) return rc;
+    }
+    return BHAH_OK;
+  }
+  // BSSN: Ricci -> RHS -> radiation/extrapolation + inner BCs on RHS -> RK update + det(gammabar)
+  // enforcement in one pass (two_blackholes_collide.py:290-313).
+  for (int s = 1; s <= 4; s++) {
+    double *in = in_s[s - 1], *k = out_s[s - 1];
+    if ((rc = bhah_hip_Ricci_eval(P, in, gf->auxevol_gfs))) return rc;
+    if ((rc = bhah_hip_rhs_eval(P, gf->auxevol_gfs, in, k))) return rc;
+    if (P->d.outer_bc == BHAH_OUTER_BC_RADIATION) {
+      if ((rc = bhah_hip_apply_bcs_outerradiation_and_inner(P, in, k))) return rc;
+    }
+    dim3 g, b;
+    P->all_dims(g, b);
+    double *next = (s == 4) ? gf->y_n_gfs : k;
+    HIP_TRY(P->k->rk_det(g, b, P->stream, k, gf->y_n_gfs, gf->y_nplus1_running_total_gfs, next, P->d_rfm0, P->d_rfm1,
+                         P->kp(dt, s)));
+    if (P->d.outer_bc == BHAH_OUTER_BC_EXTRAPOLATION) {
+      // post-RHS extrapolation BCs then det enforcement (two_blackholes_collide.py:304-307)
+      if ((rc = bhah_hip_apply_bcs_outerextrap_and_inner(P, next))) return rc;
+      if ((rc = bhah_hip_enforce_detgammabar_equals_detgammahat(P, next))) return rc;
+    }
+  }
+  return BHAH_OK;
+}
+
+int bhah_hip_MoL_step_forward_in_time(bhah_patch *P, bhah_MoL_gridfunctions *gf, double dt, double *time) {
+  if (!P || !gf || !gf->y_n_gfs || !gf->k_odd_gfs || !gf->k_even_gfs || !gf->y_nplus1_running_total_gfs ||
+      (P->naux && !gf->auxevol_gfs))
+    return fail(BHAH_ERR_ARG, "MoL gridfunctions not allocated");
+  int rc = enqueue_step(P, gf, dt);
+  if (rc) return rc;
+  if (time) *time += dt;
+  return BHAH_OK;
+}
+
+int bhah_hip_MoL_step_n(bhah_patch *P, bhah_MoL_gridfunctions *gf, double dt, int nsteps, int use_graph) {
+  if (!P || !gf || nsteps < 0) return fail(BHAH_ERR_ARG, "bad arguments");
+  if (!use_graph) {
+    for (int n = 0; n < nsteps; n++) {
+      int rc = bhah_hip_MoL_step_forward_in_time(P, gf, dt, nullptr);
+      if (rc) return rc;
+    }
+    return BHAH_OK;
+  }
+  const bool reuse = P->graph_exec && memcmp(&P->graph_gf, gf, sizeof(*gf)) == 0 && P->graph_dt == dt;
+  if (!reuse) {
+    if (P->graph_exec) {
+      hipGraphExecDestroy(P->graph_exec);
+      P->graph_exec = nullptr;
+    }
+    hipStream_t cap = P->stream;
+    bool tmp_stream = false;
+    if (!cap) {  // the null stream cannot be captured
+      HIP_TRY(hipStreamCreate(&cap));
+      tmp_stream = true;
+    }
+    hipStream_t saved = P->stream;
+    P->stream = cap;
+    HIP_TRY(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_step(P, gf, dt);
+    hipGraph_t graph;
+    hipError_t e = hipStreamEndCapture(cap, &graph);
+    P->stream = saved;
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(BHAH_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&P->graph_exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (tmp_stream) hipStreamDestroy(cap);
+    if (e != hipSuccess) return fail(BHAH_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
+    P->graph_gf = *gf;
+    P->graph_dt = dt;
+  }
+  for (int n = 0; n < nsteps; n++) HIP_TRY(hipGraphLaunch(P->graph_exec, P->stream));
+  return BHAH_OK;
+}
+
+int bhah_hip_wave_rhs_eval(bhah_patch *P, const double *in, double *rhs) {
+  if (!P || P->d.system != BHAH_SYSTEM_WAVE) return fail(BHAH_ERR_ARG, "patch is not a wave patch");
+  if (!in || !rhs) return fail(BHAH_ERR_ARG, "NULL argument");
+  dim3 g, b;
+  P->interior_dims(g, b);
+  HIP_TRY(launch_wave_rhs_fd4(g, b, P->stream, in, rhs, P->kp(0, 0)));
+  return BHAH_OK;
+}
+
+int bhah_hip_wave_apply_bcs(bhah_patch *P, double *gfs) {
+  if (!P || !gfs) return fail(BHAH_ERR_ARG, "NULL argument");
+  // Cartesian: every ghost point is a pure outer point; the outer lists follow the same
+  // face order as the example's FACE_UPDATE sweep.
+  return bhah_hip_apply_bcs_outerextrap_and_inner(P, gfs);
+}
+
+int bhah_hip_diagnostics_constraints(bhah_patch *P, const double *aux, const double *in, double *diag,
+                                     double *l2_H) {
+  if (!P || !P->k) return fail(BHAH_ERR_ARG, "patch is not a BSSN patch");
+  if (!aux || !in || !diag) return fail(BHAH_ERR_ARG, "NULL argument");
+  dim3 g, b;
+  P->interior_dims(g, b);
+  HIP_TRY(P->k->constraints(g, b, P->stream, in, aux, diag, P->d_rfm0, P->d_rfm1, P->kp(0, 0)));
+  if (l2_H) {
+    HIP_TRY(hipMemsetAsync(P->d_red, 0, 3 * sizeof(double), P->stream));
+    hipLaunchKernelGGL(l2_reduce_kernel, g, b, 0, P->stream, diag, diag, P->d_rfm0, P->d_rfm1, P->kp(0, 0),
+                       fabs(P->dxx[0] * P->dxx[1] * P->dxx[2]), P->d_red);
+    HIP_TRY(hipGetLastError());
+    double red[3];
+    HIP_TRY(hipMemcpyAsync(red, P->d_red, 3 * sizeof(double), hipMemcpyDeviceToHost, P->stream));
+    HIP_TRY(hipStreamSynchronize(P->stream));
+    l2_H[0] = sqrt(red[1] / red[0]);  // RMS over the proper volume
+    l2_H[1] = sqrt(red[1]);           // sqrt of the volume integral of H^2
+    l2_H[2] = red[0];                 // proper volume
+  }
+  return BHAH_OK;
+}
+
+}  // extern "C"

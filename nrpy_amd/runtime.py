@@ -1,0 +1,327 @@
+"""ctypes binding of ``libbhah_hip.so`` with PyTorch-ROCm tensors as device memory.
+
+``Patch`` mirrors the reference's generated-C call sequence for one grid
+(``nrpy/examples/two_blackholes_collide.py:290-313``): ``Ricci_eval``,
+``rhs_eval``, the curvilinear boundary conditions, det(gammabar) enforcement and
+``MoL_step_forward_in_time``.  Gridfunction arrays are torch float64 CUDA
+(HIP) tensors of shape (num_gfs, Nx2, Nx1, Nx0), i.e. the BHaH gf-major
+layout; their ``data_ptr()`` goes through the C-ABI unchanged.
+
+There is no CPU fallback: every call goes to the HIP library, and loading
+fails with an error if it was not built.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+from typing import Dict, Optional
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libbhah_hip.so")
+
+CARTESIAN, SPHERICAL, SINHSPHERICAL = 0, 1, 2
+SYSTEM_BSSN, SYSTEM_WAVE = 0, 1
+OUTER_BC_RADIATION, OUTER_BC_EXTRAPOLATION = 0, 1
+
+EVOL_GFS = [
+    "aDD00", "aDD01", "aDD02", "aDD11", "aDD12", "aDD22", "alpha",
+    "betU0", "betU1", "betU2", "cf",
+    "hDD00", "hDD01", "hDD02", "hDD11", "hDD12", "hDD22",
+    "lambdaU0", "lambdaU1", "lambdaU2", "trK", "vetU0", "vetU1", "vetU2",
+]
+
+
+class PatchDesc(ctypes.Structure):
+    _fields_ = [
+        ("system", ctypes.c_int),
+        ("coord", ctypes.c_int),
+        ("Nxx", ctypes.c_int * 3),
+        ("fd_order", ctypes.c_int),
+        ("symmetry_axes", ctypes.c_int),
+        ("enable_KO", ctypes.c_int),
+        ("outer_bc", ctypes.c_int),
+        ("radiation_bc_fd_order", ctypes.c_int),
+        ("grid_physical_size", ctypes.c_double),
+        ("SINHW", ctypes.c_double),
+        ("CFL_FACTOR", ctypes.c_double),
+        ("eta", ctypes.c_double),
+        ("KO_strength_gauge", ctypes.c_double),
+        ("KO_strength_nongauge", ctypes.c_double),
+        ("wavespeed", ctypes.c_double),
+    ]
+
+
+class MoLGridfunctions(ctypes.Structure):
+    _fields_ = [
+        ("y_n_gfs", ctypes.c_void_p),
+        ("y_nplus1_running_total_gfs", ctypes.c_void_p),
+        ("k_odd_gfs", ctypes.c_void_p),
+        ("k_even_gfs", ctypes.c_void_p),
+        ("auxevol_gfs", ctypes.c_void_p),
+    ]
+
+
+class PatchInfo(ctypes.Structure):
+    _fields_ = [
+        ("Nxx", ctypes.c_int * 3),
+        ("NGHOSTS", ctypes.c_int),
+        ("Nxx_plus_2NGHOSTS", ctypes.c_int * 3),
+        ("num_evol_gfs", ctypes.c_int),
+        ("num_auxevol_gfs", ctypes.c_int),
+        ("Ntot", ctypes.c_int64),
+        ("xxmin", ctypes.c_double * 3),
+        ("xxmax", ctypes.c_double * 3),
+        ("dxx", ctypes.c_double * 3),
+        ("invdxx", ctypes.c_double * 3),
+        ("dt", ctypes.c_double),
+        ("num_inner_boundary_points", ctypes.c_int),
+        ("num_pure_outer_boundary_points", ctypes.c_int),
+    ]
+
+
+# Every entry point declared in include/bhah_hip.h, with its argument types.
+_P = ctypes.c_void_p
+_D = ctypes.c_double
+_I = ctypes.c_int
+SIGNATURES: Dict[str, tuple] = {
+    "bhah_hip_last_error": (ctypes.c_char_p, []),
+    "bhah_hip_abi_version": (_I, []),
+    "bhah_hip_patch_create": (_I, [ctypes.POINTER(PatchDesc), ctypes.POINTER(_P)]),
+    "bhah_hip_patch_destroy": (_I, [_P]),
+    "bhah_hip_patch_get_info": (_I, [_P, ctypes.POINTER(PatchInfo)]),
+    "bhah_hip_patch_set_stream": (_I, [_P, _P]),
+    "bhah_hip_synchronize": (_I, [_P]),
+    "bhah_hip_patch_get_xx": (_I, [_P, _P, _P, _P]),
+    "bhah_hip_patch_get_inner_bc": (_I, [_P, _P, _P, _P]),
+    "bhah_hip_patch_get_outer_bc_count": (_I, [_P, _I, _I]),
+    "bhah_hip_patch_get_outer_bc": (_I, [_P, _I, _I, _P]),
+    "bhah_hip_rhs_eval": (_I, [_P, _P, _P, _P]),
+    "bhah_hip_Ricci_eval": (_I, [_P, _P, _P]),
+    "bhah_hip_enforce_detgammabar_equals_detgammahat": (_I, [_P, _P]),
+    "bhah_hip_apply_bcs_outerradiation_and_inner": (_I, [_P, _P, _P]),
+    "bhah_hip_apply_bcs_outerextrap_and_inner": (_I, [_P, _P]),
+    "bhah_hip_apply_bcs_inner_only": (_I, [_P, _P]),
+    "bhah_hip_MoL_malloc_intermediate_stage_gfs": (_I, [_P, ctypes.POINTER(MoLGridfunctions)]),
+    "bhah_hip_MoL_free": (_I, [_P, ctypes.POINTER(MoLGridfunctions)]),
+    "bhah_hip_MoL_step_forward_in_time": (_I, [_P, ctypes.POINTER(MoLGridfunctions), _D, ctypes.POINTER(_D)]),
+    "bhah_hip_MoL_step_n": (_I, [_P, ctypes.POINTER(MoLGridfunctions), _D, _I, _I]),
+    "bhah_hip_wave_rhs_eval": (_I, [_P, _P, _P]),
+    "bhah_hip_wave_apply_bcs": (_I, [_P, _P]),
+    "bhah_hip_diagnostics_constraints": (_I, [_P, _P, _P, _P, ctypes.POINTER(_D)]),
+}
+
+_LIB: Optional[ctypes.CDLL] = None
+
+
+class BHaHError(RuntimeError):
+    pass
+
+
+def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load the HIP library (no fallback: raises if it is absent)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise BHaHError(f"{path} not found: build it with `python -m nrpy_amd.build`")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = lib
+    return lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        msg = load_library().bhah_hip_last_error().decode()
+        raise BHaHError(f"bhah_hip error {rc}: {msg}")
+
+
+@dataclass
+class PatchConfig:
+    """Python view of ``bhah_patch_desc`` with the reference's defaults."""
+
+    system: int = SYSTEM_BSSN
+    coord: int = SPHERICAL
+    Nxx: tuple = (72, 12, 2)
+    fd_order: int = 4
+    symmetry_axes: int = 4  # phi
+    enable_KO: bool = False
+    outer_bc: int = OUTER_BC_RADIATION
+    radiation_bc_fd_order: int = 4
+    grid_physical_size: float = 7.5
+    SINHW: float = 0.2
+    CFL_FACTOR: float = 0.5
+    eta: float = 2.0
+    KO_strength_gauge: float = 0.3
+    KO_strength_nongauge: float = 0.3
+    wavespeed: float = 1.0
+
+    def to_c(self) -> PatchDesc:
+        d = PatchDesc()
+        d.system, d.coord = self.system, self.coord
+        for i in range(3):
+            d.Nxx[i] = int(self.Nxx[i])
+        d.fd_order = self.fd_order
+        d.symmetry_axes = self.symmetry_axes
+        d.enable_KO = int(self.enable_KO)
+        d.outer_bc = self.outer_bc
+        d.radiation_bc_fd_order = self.radiation_bc_fd_order
+        d.grid_physical_size = self.grid_physical_size
+        d.SINHW = self.SINHW
+        d.CFL_FACTOR = self.CFL_FACTOR
+        d.eta = self.eta
+        d.KO_strength_gauge = self.KO_strength_gauge
+        d.KO_strength_nongauge = self.KO_strength_nongauge
+        d.wavespeed = self.wavespeed
+        return d
+
+
+def _ptr(t) -> int:
+    if t is None:
+        return None
+    import torch
+
+    if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+        raise BHaHError("gridfunction tensors must be contiguous float64 device tensors")
+    return t.data_ptr()
+
+
+class Patch:
+    """One grid patch: geometry, boundary structs and kernels on the current device."""
+
+    def __init__(self, cfg: PatchConfig, stream=None):
+        import torch
+
+        self.lib = load_library()
+        self.cfg = cfg
+        self._desc = cfg.to_c()
+        h = ctypes.c_void_p()
+        _check(self.lib.bhah_hip_patch_create(ctypes.byref(self._desc), ctypes.byref(h)))
+        self.handle = h
+        info = PatchInfo()
+        _check(self.lib.bhah_hip_patch_get_info(self.handle, ctypes.byref(info)))
+        self.info = info
+        self.NG = info.NGHOSTS
+        self.Nx = tuple(info.Nxx_plus_2NGHOSTS)
+        self.Ntot = int(info.Ntot)
+        self.ngfs = info.num_evol_gfs
+        self.naux = info.num_auxevol_gfs
+        self.dt = info.dt
+        self.dxx = tuple(info.dxx)
+        s = stream if stream is not None else torch.cuda.current_stream()
+        _check(self.lib.bhah_hip_patch_set_stream(self.handle, ctypes.c_void_p(s.cuda_stream)))
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            self.lib.bhah_hip_patch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- allocation helpers
+    def new_gfs(self, n: Optional[int] = None):
+        import torch
+
+        n = self.ngfs if n is None else n
+        return torch.zeros((n, self.Nx[2], self.Nx[1], self.Nx[0]), dtype=torch.float64, device="cuda")
+
+    def xx(self):
+        import numpy as np
+
+        x = [np.zeros(n) for n in self.Nx]
+        _check(self.lib.bhah_hip_patch_get_xx(self.handle, *[a.ctypes.data for a in x]))
+        return x
+
+    def inner_bc(self):
+        import numpy as np
+
+        n = self.info.num_inner_boundary_points
+        dst, src = np.zeros(n, np.int64), np.zeros(n, np.int64)
+        par = np.zeros((n, 10), np.int8)
+        if n:
+            _check(self.lib.bhah_hip_patch_get_inner_bc(self.handle, dst.ctypes.data, src.ctypes.data, par.ctypes.data))
+        return dst, src, par
+
+    def outer_bc(self, gz: int, dirn: int):
+        import numpy as np
+
+        n = self.lib.bhah_hip_patch_get_outer_bc_count(self.handle, gz, dirn)
+        if n < 0:
+            _check(n)
+        out = np.zeros((n, 6), np.int16)
+        if n:
+            _check(self.lib.bhah_hip_patch_get_outer_bc(self.handle, gz, dirn, out.ctypes.data))
+        return out
+
+    # ---- reference API counterparts
+    def rhs_eval(self, auxevol, in_gfs, rhs):
+        _check(self.lib.bhah_hip_rhs_eval(self.handle, _ptr(auxevol), _ptr(in_gfs), _ptr(rhs)))
+
+    def Ricci_eval(self, in_gfs, auxevol):
+        _check(self.lib.bhah_hip_Ricci_eval(self.handle, _ptr(in_gfs), _ptr(auxevol)))
+
+    def enforce_detgammabar_equals_detgammahat(self, gfs):
+        _check(self.lib.bhah_hip_enforce_detgammabar_equals_detgammahat(self.handle, _ptr(gfs)))
+
+    def apply_bcs_outerradiation_and_inner(self, gfs, rhs):
+        _check(self.lib.bhah_hip_apply_bcs_outerradiation_and_inner(self.handle, _ptr(gfs), _ptr(rhs)))
+
+    def apply_bcs_outerextrap_and_inner(self, gfs):
+        _check(self.lib.bhah_hip_apply_bcs_outerextrap_and_inner(self.handle, _ptr(gfs)))
+
+    def apply_bcs_inner_only(self, gfs):
+        _check(self.lib.bhah_hip_apply_bcs_inner_only(self.handle, _ptr(gfs)))
+
+    def wave_rhs_eval(self, in_gfs, rhs):
+        _check(self.lib.bhah_hip_wave_rhs_eval(self.handle, _ptr(in_gfs), _ptr(rhs)))
+
+    def wave_apply_bcs(self, gfs):
+        _check(self.lib.bhah_hip_wave_apply_bcs(self.handle, _ptr(gfs)))
+
+    def diagnostics_constraints(self, auxevol, in_gfs, diag):
+        out = (ctypes.c_double * 3)()
+        _check(self.lib.bhah_hip_diagnostics_constraints(self.handle, _ptr(auxevol), _ptr(in_gfs), _ptr(diag), out))
+        return out[0], out[1], out[2]
+
+    def synchronize(self):
+        _check(self.lib.bhah_hip_synchronize(self.handle))
+
+
+class MoLState:
+    """MoL gridfunctions of one patch (MoL_malloc_intermediate_stage_gfs), torch-owned."""
+
+    def __init__(self, patch: Patch, y_n=None):
+        self.patch = patch
+        self.y_n = y_n if y_n is not None else patch.new_gfs()
+        self.y_run = patch.new_gfs()
+        self.k_odd = patch.new_gfs()
+        self.k_even = patch.new_gfs()
+        self.aux = patch.new_gfs(max(patch.naux, 1))
+        self.c = MoLGridfunctions(
+            _ptr(self.y_n), _ptr(self.y_run), _ptr(self.k_odd), _ptr(self.k_even), _ptr(self.aux)
+        )
+        self.time = 0.0
+        self.nn = 0
+
+    def step(self, dt: Optional[float] = None):
+        """MoL_step_forward_in_time: one RK4 step."""
+        dt = self.patch.dt if dt is None else dt
+        t = ctypes.c_double(self.time)
+        _check(self.patch.lib.bhah_hip_MoL_step_forward_in_time(self.patch.handle, ctypes.byref(self.c), dt, ctypes.byref(t)))
+        self.nn += 1
+        self.time = self.nn * dt
+
+    def steps(self, n: int, dt: Optional[float] = None, use_graph: bool = True):
+        dt = self.patch.dt if dt is None else dt
+        _check(self.patch.lib.bhah_hip_MoL_step_n(self.patch.handle, ctypes.byref(self.c), dt, n, int(use_graph)))
+        self.nn += n
+        self.time = self.nn * dt

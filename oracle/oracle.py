@@ -1,0 +1,212 @@
+"""ORACLE / TEST INFRASTRUCTURE ONLY: ctypes wrapper of ``liboracle_bhah.so``.
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline leg use
+this module, always as the checker.  Arrays are numpy float64 in the BHaH
+layout (num_gfs, Nx2, Nx1, Nx0).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle_bhah.so")
+
+CARTESIAN, SPHERICAL, SINHSPHERICAL = 0, 1, 2
+PARITY = [4, 5, 6, 7, 8, 9, 0, 1, 2, 3, 0, 4, 5, 6, 7, 8, 9, 1, 2, 3, 0, 1, 2, 3]
+_lib = None
+
+
+class OGrid(ctypes.Structure):
+    _fields_ = [
+        ("coord", ctypes.c_int),
+        ("Nxx", ctypes.c_int * 3),
+        ("NG", ctypes.c_int),
+        ("Nx", ctypes.c_int * 3),
+        ("xxmin", ctypes.c_double * 3),
+        ("xxmax", ctypes.c_double * 3),
+        ("dxx", ctypes.c_double * 3),
+        ("invdxx", ctypes.c_double * 3),
+        ("p0", ctypes.c_double),
+        ("p1", ctypes.c_double),
+        ("fd_order", ctypes.c_int),
+    ]
+
+
+def build() -> str:
+    """Compile the oracle with its Makefile (gcc)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(LIB_PATH)
+        P, I, D, L = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_long
+        sig = {
+            "og_grid_init": (None, [P, I, I, I, I, I, I, D, D]),
+            "og_xx": (None, [P, P, P, P]),
+            "og_rfm": (None, [P, P, P, P, P]),
+            "og_bssn_rhs": (None, [P, I, P, P, P, P, P, P]),
+            "og_ricci": (None, [P, I, P, P, P, P]),
+            "og_enforce_detgamma": (None, [P, I, P, P, P]),
+            "og_constraints": (None, [P, I, P, P, P, P, P]),
+            "og_wave_rhs": (None, [P, D, P, P]),
+            "og_bcstruct_new": (P, [P, P, P, P]),
+            "og_bc_num_inner": (I, [P]),
+            "og_bc_inner": (None, [P, P, P, P]),
+            "og_bc_num_outer": (I, [P, I, I]),
+            "og_bc_outer": (None, [P, I, I, P]),
+            "og_bcstruct_free": (None, [P, I]),
+            "og_apply_bcs_inner_only": (None, [P, P, I, P, P]),
+            "og_apply_bcs_outerextrap_and_inner": (None, [P, P, I, P, P]),
+            "og_apply_bcs_outerradiation_and_inner": (None, [P, P, P, I, P, P, P, I, P, P]),
+            "og_wave_apply_bcs": (None, [P, P]),
+            "og_rk_substep": (None, [I, L, D, P, P, P]),
+            "og_bssn_mol_step": (None, [P, I, P, P, P, P, P, I, D, P, P, P, P, P]),
+            "og_wave_mol_step": (None, [P, D, D, P, P, P, P]),
+            "og_cfl_dt": (D, [P, P, P, P, D]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(_lib, name)
+            fn.restype = res
+            fn.argtypes = args
+    return _lib
+
+
+def _p(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    assert a.dtype in (np.float64, np.int64, np.int8, np.int16, np.int32) and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data
+
+
+class OracleGrid:
+    """Grid + rfm arrays + boundary structs of the CPU restatement."""
+
+    def __init__(self, coord: int, Nxx, fd_order: int, size: float, sinhw: float = 0.2,
+                 variant: int = 0, NG: Optional[int] = None, with_bc: bool = True):
+        L = lib()
+        self.g = OGrid()
+        if NG is None:
+            NG = fd_order // 2 + (0 if coord == CARTESIAN else 1)
+        L.og_grid_init(ctypes.byref(self.g), coord, int(Nxx[0]), int(Nxx[1]), int(Nxx[2]), NG, fd_order, size, sinhw)
+        self.coord, self.variant, self.NG = coord, variant, NG
+        self.Nx = tuple(self.g.Nx)
+        self.Ntot = self.Nx[0] * self.Nx[1] * self.Nx[2]
+        self.xx = [np.zeros(n) for n in self.Nx]
+        L.og_xx(ctypes.byref(self.g), *[_p(a) for a in self.xx])
+        self.rfm0 = np.zeros(4 * self.Nx[0])
+        self.rfm1 = np.zeros(3 * self.Nx[1])
+        L.og_rfm(ctypes.byref(self.g), _p(self.xx[0]), _p(self.xx[1]), _p(self.rfm0), _p(self.rfm1))
+        self.bc = None
+        if with_bc and coord != CARTESIAN:
+            self.bc = L.og_bcstruct_new(ctypes.byref(self.g), *[_p(a) for a in self.xx])
+
+    def __del__(self):
+        if getattr(self, "bc", None):
+            lib().og_bcstruct_free(self.bc, self.NG)
+            self.bc = None
+
+    def gfs(self, n: int = 24) -> np.ndarray:
+        return np.zeros((n, self.Nx[2], self.Nx[1], self.Nx[0]))
+
+    def params(self, eta=2.0, ko_g=0.3, ko_ng=0.3, wavespeed=1.0) -> np.ndarray:
+        return np.array([eta, ko_g, ko_ng, wavespeed])
+
+    # ---- pointwise-kernel loops
+    def rhs(self, in_gfs, aux, params) -> np.ndarray:
+        out = self.gfs(24)
+        lib().og_bssn_rhs(ctypes.byref(self.g), self.variant, _p(params), _p(self.rfm0), _p(self.rfm1),
+                          _p(aux), _p(in_gfs), _p(out))
+        return out
+
+    def ricci(self, in_gfs) -> np.ndarray:
+        aux = self.gfs(6)
+        lib().og_ricci(ctypes.byref(self.g), self.variant, _p(self.rfm0), _p(self.rfm1), _p(in_gfs), _p(aux))
+        return aux
+
+    def enforce_detgamma(self, gfs) -> np.ndarray:
+        out = gfs.copy()
+        lib().og_enforce_detgamma(ctypes.byref(self.g), self.variant, _p(self.rfm0), _p(self.rfm1), _p(out))
+        return out
+
+    def constraints(self, in_gfs, aux) -> np.ndarray:
+        out = self.gfs(4)
+        lib().og_constraints(ctypes.byref(self.g), self.variant, _p(self.rfm0), _p(self.rfm1), _p(aux),
+                             _p(in_gfs), _p(out))
+        return out
+
+    def wave_rhs(self, in_gfs, wavespeed=1.0) -> np.ndarray:
+        out = self.gfs(2)
+        lib().og_wave_rhs(ctypes.byref(self.g), wavespeed, _p(in_gfs), _p(out))
+        return out
+
+    # ---- boundary conditions
+    def inner_bc(self):
+        L = lib()
+        n = L.og_bc_num_inner(self.bc)
+        dst, src = np.zeros(n, np.int64), np.zeros(n, np.int64)
+        par = np.zeros((n, 10), np.int8)
+        if n:
+            L.og_bc_inner(self.bc, _p(dst), _p(src), _p(par))
+        return dst, src, par
+
+    def outer_bc(self, gz, d):
+        L = lib()
+        n = L.og_bc_num_outer(self.bc, gz, d)
+        out = np.zeros((n, 6), np.int16)
+        if n:
+            L.og_bc_outer(self.bc, gz, d, _p(out))
+        return out
+
+    def apply_bcs_inner_only(self, gfs, parity=None):
+        parity = np.array(parity if parity is not None else PARITY[: gfs.shape[0]], dtype=np.int32)
+        lib().og_apply_bcs_inner_only(ctypes.byref(self.g), self.bc, gfs.shape[0], _p(parity), _p(gfs))
+
+    def apply_bcs_outerextrap_and_inner(self, gfs, parity=None):
+        parity = np.array(parity if parity is not None else PARITY[: gfs.shape[0]], dtype=np.int32)
+        lib().og_apply_bcs_outerextrap_and_inner(ctypes.byref(self.g), self.bc, gfs.shape[0], _p(parity), _p(gfs))
+
+    def apply_bcs_outerradiation_and_inner(self, gfs, rhs, rad_order=4):
+        parity = np.array(PARITY, dtype=np.int32)
+        ws = np.ones(24)
+        fi = np.zeros(24)
+        ws[6] = np.sqrt(2.0)
+        fi[6] = 1.0
+        fi[10] = 1.0
+        lib().og_apply_bcs_outerradiation_and_inner(ctypes.byref(self.g), self.bc, _p(self.xx[0]), 24, _p(parity),
+                                                    _p(ws), _p(fi), rad_order, _p(gfs), _p(rhs))
+
+    def wave_apply_bcs(self, gfs):
+        lib().og_wave_apply_bcs(ctypes.byref(self.g), _p(gfs))
+
+    def cfl_dt(self, CFL: float = 0.5) -> float:
+        return lib().og_cfl_dt(ctypes.byref(self.g), *[_p(a) for a in self.xx], CFL)
+
+    # ---- time stepping
+    def bssn_mol_step(self, state: dict, params, dt: float, rad_order: int = 4) -> None:
+        lib().og_bssn_mol_step(ctypes.byref(self.g), self.variant, _p(params), _p(self.rfm0), _p(self.rfm1),
+                               self.bc, _p(self.xx[0]), rad_order, dt, _p(state["y_n"]), _p(state["y_run"]),
+                               _p(state["k_odd"]), _p(state["k_even"]), _p(state["aux"]))
+
+    def new_state(self, y_n: np.ndarray, nvars: int = 24) -> dict:
+        return {
+            "y_n": np.ascontiguousarray(y_n.copy()),
+            "y_run": self.gfs(nvars),
+            "k_odd": self.gfs(nvars),
+            "k_even": self.gfs(nvars),
+            "aux": self.gfs(6),
+        }
+
+    def wave_mol_step(self, state: dict, dt: float, wavespeed: float = 1.0) -> None:
+        lib().og_wave_mol_step(ctypes.byref(self.g), wavespeed, dt, _p(state["y_n"]), _p(state["y_run"]),
+                               _p(state["k_odd"]), _p(state["k_even"]))
