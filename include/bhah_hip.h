@@ -160,7 +160,8 @@ int bhah_hip_wave_apply_bcs(bhah_patch *patch, double *gfs);
 /* Diagnostics (nrpy/infrastructures/BHaH/diagnostics/diagnostics.py:46-270; constraints
  * nrpy/equations/general_relativity/BSSN_constraints.py:82-157; volume integrals
  * diagnostics_volume_integration_helpers.h:396-476): Hamiltonian constraint H and momentum
- * constraint MU^i at interior points into diag (device, 4*Ntot: H, MU0, MU1, MU2).  If l2_H is
+ * constraint MU^i and M^2 = gammabar_ij MU^i MU^j e^{4 phi} at interior points into diag (device,
+ * 5*Ntot: H, MU0, MU1, MU2, MSQUARED).  If l2_H is
  * not NULL it receives {RMS of H over the proper volume, sqrt(integral of H^2 dV), volume} with
  * dV = sqrt(det ghat) dxx0 dxx1 dxx2 (synchronises the stream).  Needs RbarDD in auxevol_gfs
  * (call bhah_hip_Ricci_eval on in_gfs first). */

@@ -162,6 +162,23 @@ def stencil_flops(d: fd.Deriv, fd_order: int) -> int:
     raise ValueError(d.op)
 
 
+def _split_one(args):
+    g, e, lin_names = args
+    lin = [sp.Symbol(n, real=True) for n in lin_names]
+    core = e.xreplace({s: 0 for s in lin})
+    terms = []
+    for s in lin:
+        if s in e.free_symbols:
+            c = sp.diff(e, s)
+            if c.free_symbols & set(lin):
+                raise ValueError(f"{s} enters {g} nonlinearly")
+            terms.append((str(s), c))
+    return g, core, terms
+
+
+_SPLIT_CACHE: Dict[str, tuple] = {}
+
+
 def split_linear(spec: specs.Spec):
     """Split each output into core + sum(coef * s) over upwind/KO derivatives s.
 
@@ -170,18 +187,25 @@ def split_linear(spec: specs.Spec):
     coefficient that depends only on the shift and the reference metric.
     Returns (cores, {output_gf: [(deriv, coef), ...]}).
     """
+    if spec.name in _SPLIT_CACHE:
+        return _SPLIT_CACHE[spec.name]
     _, _, derivs, _, _ = spec.classify()
-    lin = {sp.Symbol(d.symbol, real=True): d for d in derivs if d.op in ("dupD", "dKOD")}
+    dmap = {d.symbol: d for d in derivs if d.op in ("dupD", "dKOD")}
+    names = sorted(dmap)
+    jobs = [(g, e, names) for g, e in spec.outputs]
+    if names:
+        import multiprocessing as mp
+
+        with mp.get_context("fork").Pool(min(len(jobs), os.cpu_count() or 1)) as pool:
+            res = pool.map(_split_one, jobs)
+    else:
+        res = [(g, e, []) for g, e, _ in jobs]
     cores, terms = [], {}
-    for g, e in spec.outputs:
-        core = e.xreplace({s: 0 for s in lin})
+    for g, core, tl in res:
         cores.append(core)
-        for s, d in lin.items():
-            if s in e.free_symbols:
-                c = sp.diff(e, s)
-                if c.free_symbols & set(lin):
-                    raise ValueError(f"{s} enters {g} nonlinearly")
-                terms.setdefault(g, []).append((d, c))
+        if tl:
+            terms[g] = [(dmap[n], c) for n, c in tl]
+    _SPLIT_CACHE[spec.name] = (cores, terms)
     return cores, terms
 
 

@@ -446,14 +446,14 @@ def detgamma_enforce_exprs(coord: str) -> Dict[str, sp.Expr]:
     return out
 
 
-def constraint_exprs(coord: str) -> Dict[str, sp.Expr]:
-    """Hamiltonian constraint H and M^i M_i-free momentum constraint MU^i.
+def constraint_exprs(coord: str, ricci_symbolic: bool = True) -> Dict[str, sp.Expr]:
+    """Hamiltonian constraint H, momentum constraint MU^i and M^2.
 
-    Follows ``BSSN_constraints.py:82-157`` with RbarDD symbolic (read from the
-    auxiliary Ricci gridfunctions).
+    Follows ``BSSN_constraints.py:82-183``; with ``ricci_symbolic`` RbarDD is read
+    from the auxiliary Ricci gridfunctions, otherwise expanded from hDD.
     """
     rfm = reference_metric(coord)
-    Bq = BSSNQuantities(rfm, ricci_symbolic=True)
+    Bq = BSSNQuantities(rfm, ricci_symbolic=ricci_symbolic)
     I3 = range(3)
     gUU = Bq.gammabarUU
     H = R(2, 3) * Bq.trK**2
@@ -476,4 +476,6 @@ def constraint_exprs(coord: str) -> Dict[str, sp.Expr]:
     for i, j, k, l in T.idx(4):
         MU[i] += gUU[i][k] * gUU[j][l] * AdB[k][l][j]
     MU = [m * Bq.exp_m4phi for m in MU]
-    return {"H": H, "MU0": MU[0], "MU1": MU[1], "MU2": MU[2]}
+    # M^2 = gammabar_ij MU^i MU^j / e^{-4 phi} (BSSN_constraints.py:177-182)
+    Msq = sum(Bq.gammabarDD[i][j] * MU[i] * MU[j] for i in I3 for j in I3) / Bq.exp_m4phi
+    return {"H": H, "MU0": MU[0], "MU1": MU[1], "MU2": MU[2], "MSQUARED": Msq}

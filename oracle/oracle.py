@@ -140,7 +140,7 @@ class OracleGrid:
         return out
 
     def constraints(self, in_gfs, aux) -> np.ndarray:
-        out = self.gfs(4)
+        out = self.gfs(5)
         lib().og_constraints(ctypes.byref(self.g), self.variant, _p(self.rfm0), _p(self.rfm1), _p(aux),
                              _p(in_gfs), _p(out))
         return out

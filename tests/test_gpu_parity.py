@@ -114,7 +114,7 @@ def test_bcs_and_detgamma_per_call(hip, oracle_lib, case):
     det_d = torch.from_numpy(y.copy()).cuda()
     P.enforce_detgammabar_equals_detgammahat(det_d)
     er = rel_err(det_d.cpu().numpy(), det_o)
-    assert er.max() < 1e-13, er
+    assert er.max() < 1e-12, er
 
 
 @pytest.mark.parametrize("case", list(CASES))
@@ -150,7 +150,7 @@ def test_constraints_match_oracle(hip, oracle_lib, case):
     y = state(O, 5)
     aux = O.ricci(y)
     c_o = O.constraints(y, aux)
-    diag = P.new_gfs(4)
+    diag = P.new_gfs(5)
     rms, l2, vol = P.diagnostics_constraints(torch.from_numpy(aux).cuda(), torch.from_numpy(y).cuda(), diag)
     NG = O.NG
     er = rel_err(interior(diag.cpu().numpy(), NG), interior(c_o, NG))
