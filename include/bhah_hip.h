@@ -48,6 +48,7 @@ enum {
 enum { BHAH_CARTESIAN = 0, BHAH_SPHERICAL = 1, BHAH_SINHSPHERICAL = 2 };
 enum { BHAH_SYSTEM_BSSN = 0, BHAH_SYSTEM_WAVE = 1 };
 enum { BHAH_OUTER_BC_RADIATION = 0, BHAH_OUTER_BC_EXTRAPOLATION = 1 };
+enum { BHAH_KERNEL_RHS = 0, BHAH_KERNEL_RICCI = 1 };
 
 #define BHAH_NUM_EVOL_GFS_BSSN 24
 #define BHAH_NUM_AUXEVOL_GFS_BSSN 6
@@ -151,6 +152,20 @@ int bhah_hip_MoL_step_forward_in_time(bhah_patch *patch, bhah_MoL_gridfunctions 
 /* nsteps steps; with use_graph != 0 the four substeps are captured once in a hipGraph and
  * replayed (no HIP API work per step beyond the graph launch). */
 int bhah_hip_MoL_step_n(bhah_patch *patch, bhah_MoL_gridfunctions *gf, double dt, int nsteps, int use_graph);
+
+/* Kernel schedules.  rhs_eval and Ricci_eval each come in several generated
+ * schedules (partitions of the outputs into kernels x register budgets); all
+ * compute the same values.  The autotuner (nrpy_amd/autotune.py) times them on
+ * the device and stores the fastest per workload.  No reference counterpart:
+ * the reference emits one kernel per function
+ * (nrpy/helpers/parallelization/utilities.py:108-196). */
+int bhah_hip_patch_num_schedules(const bhah_patch *patch, int kind);
+const char *bhah_hip_patch_schedule_name(const bhah_patch *patch, int kind, int index);
+int bhah_hip_patch_select_schedule(bhah_patch *patch, int kind, int index);
+int bhah_hip_patch_get_schedule(const bhah_patch *patch, int kind);
+/* Interior thread-block shape (tile_x * tile_y = 256, tile_x along xx0); replaces the
+ * reference's BHAH_THREADS_IN_{X,Y,Z}_DIR_* macros (nrpy/helpers/parallelization/gpu_kernel.py:131-158). */
+int bhah_hip_patch_set_tile(bhah_patch *patch, int tile_x, int tile_y);
 
 /* Wave equation: rhs_eval (nrpy/examples/wave_equation_cartesian.py:318-357) and quadratic
  * extrapolation apply_bcs (wave_equation_cartesian.py:374-431). */

@@ -1,0 +1,165 @@
+"""On-device autotuner for the BSSN kernel schedules and thread-block tiles.
+
+For one workload (coordinate system, grid, FD order, KO) it times every
+generated schedule of ``rhs_eval`` and ``Ricci_eval`` (output partitions x
+register budgets) and every interior tile shape with HIP events on the
+patch's stream, on Brill-Lindquist data perturbed so all terms are live, and
+stores the fastest choice in ``nrpy_amd/autotune/<build>_<N0>x<N1>x<N2>.json``.
+``Patch`` applies a stored choice automatically (see ``apply_tuned``).
+
+    python -m nrpy_amd.autotune --config c3
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+from typing import Dict, Optional
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TABLE_DIR = os.path.join(HERE, "autotune")
+TILES = [(64, 4), (128, 2), (256, 1), (32, 8)]
+
+
+def build_tag(cfg) -> str:
+    """Name of the generated kernel build serving a patch configuration."""
+    coord = {1: "Spherical", 2: "SinhSpherical", 0: "Cartesian"}[cfg.coord]
+    axes = "".join(str(d) for d in range(3) if cfg.symmetry_axes & (1 << d)) or "none"
+    return f"{coord}_sym{axes}_KO{int(bool(cfg.enable_KO))}_fd{cfg.fd_order}"
+
+
+def table_path(cfg) -> str:
+    n = "x".join(str(int(v)) for v in cfg.Nxx)
+    return os.path.join(TABLE_DIR, f"{build_tag(cfg)}_{n}.json")
+
+
+def load_choice(cfg) -> Optional[Dict]:
+    """Stored choice for this exact grid, else the one for the nearest grid of the same build."""
+    p = table_path(cfg)
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f)
+    if not os.path.isdir(TABLE_DIR):
+        return None
+    tag = build_tag(cfg)
+    npts = cfg.Nxx[0] * cfg.Nxx[1] * cfg.Nxx[2]
+    best, bestd = None, None
+    for fn in os.listdir(TABLE_DIR):
+        if fn.startswith(tag + "_") and fn.endswith(".json"):
+            with open(os.path.join(TABLE_DIR, fn)) as f:
+                d = json.load(f)
+            n = d["Nxx"][0] * d["Nxx"][1] * d["Nxx"][2]
+            dist = abs(n - npts) / max(n, npts)
+            if bestd is None or dist < bestd:
+                best, bestd = d, dist
+    return best
+
+
+def apply_tuned(patch) -> Optional[Dict]:
+    """Select the stored schedules and tile for ``patch`` (no-op without a table)."""
+    from .runtime import KERNEL_RHS, KERNEL_RICCI, SYSTEM_BSSN
+
+    if patch.cfg.system != SYSTEM_BSSN:
+        return None
+    d = load_choice(patch.cfg)
+    if not d:
+        return None
+    if d["rhs"] in patch.schedules(KERNEL_RHS):
+        patch.select_schedule(KERNEL_RHS, d["rhs"])
+    if d["ricci"] in patch.schedules(KERNEL_RICCI):
+        patch.select_schedule(KERNEL_RICCI, d["ricci"])
+    patch.set_tile(*d["tile"])
+    return d
+
+
+def _time(fn, stream, reps: int) -> float:
+    import torch
+
+    fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def tune(cfg, reps: int = 5, write: bool = True, log=print) -> Dict:
+    import numpy as np
+    import torch
+
+    from . import initial_data as ID
+    from .runtime import KERNEL_RHS, KERNEL_RICCI, Patch
+
+    P = Patch(cfg)
+    xx = P.xx()
+    y = ID.brill_lindquist(cfg.coord, xx, cfg.grid_physical_size, cfg.SINHW)
+    y = ID.smooth_perturbation(y, xx, seed=1)
+    yd = torch.from_numpy(np.ascontiguousarray(y)).cuda()
+    aux = P.new_gfs(6)
+    rhs = P.new_gfs()
+    stream = torch.cuda.current_stream()
+    timings: Dict[str, float] = {}
+    best = {}
+    for kind, label in ((KERNEL_RICCI, "ricci"), (KERNEL_RHS, "rhs")):
+        fn = (lambda: P.Ricci_eval(yd, aux)) if kind == KERNEL_RICCI else (lambda: P.rhs_eval(aux, yd, rhs))
+        res = {}
+        for name in P.schedules(kind):
+            P.select_schedule(kind, name)
+            res[name] = _time(fn, stream, reps)
+            timings[f"{label}:{name}"] = res[name]
+            log(f"{label} {name}: {res[name]:.3f} ms")
+        best[label] = min(res, key=res.get)
+        P.select_schedule(kind, best[label])
+    tile_res = {}
+    for t in TILES:
+        P.set_tile(*t)
+        tile_res[t] = _time(lambda: (P.Ricci_eval(yd, aux), P.rhs_eval(aux, yd, rhs)), stream, reps)
+        timings[f"tile:{t[0]}x{t[1]}"] = tile_res[t]
+        log(f"tile {t}: {tile_res[t]:.3f} ms (ricci+rhs)")
+    tile = min(tile_res, key=tile_res.get)
+    out = {
+        "build": build_tag(cfg),
+        "Nxx": list(int(v) for v in cfg.Nxx),
+        "rhs": best["rhs"],
+        "ricci": best["ricci"],
+        "tile": list(tile),
+        "timings_ms": timings,
+        "device": torch.cuda.get_device_name(0),
+    }
+    if write:
+        os.makedirs(TABLE_DIR, exist_ok=True)
+        with open(table_path(cfg), "w") as f:
+            json.dump(out, f, indent=1, sort_keys=True)
+    P.close()
+    return out
+
+
+def main() -> None:
+    import sys
+
+    sys.path.insert(0, os.path.dirname(HERE))
+    from bench import CONFIGS
+    from .runtime import PatchConfig
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--config", default="c3")
+    ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--out", default=None, help="also copy the table here")
+    a = ap.parse_args()
+    c = CONFIGS[a.config]
+    cfg = PatchConfig(coord=c["coord"], Nxx=c["Nxx"], fd_order=c["fd_order"], symmetry_axes=c["sym"],
+                      enable_KO=c["KO"], grid_physical_size=c["size"], SINHW=c["sinhw"], eta=2.0)
+    out = tune(cfg, a.reps)
+    if a.out:
+        os.makedirs(os.path.dirname(a.out), exist_ok=True)
+        with open(a.out, "w") as f:
+            json.dump(out, f, indent=1, sort_keys=True)
+    print(json.dumps({k: out[k] for k in ("build", "Nxx", "rhs", "ricci", "tile")}))
+
+
+if __name__ == "__main__":
+    main()

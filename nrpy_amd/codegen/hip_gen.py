@@ -227,7 +227,7 @@ def _emit_defs(spec: specs.Spec, fd_order: int):
 
 
 def _ordered_body(spec: specs.Spec, fd_order: int, out_lvalues: List[str], rationals: Dict[str, str],
-                  stream_linear: bool = True) -> str:
+                  stream_linear: bool = True, order: str = "dfs", barriers: bool = True) -> str:
     """Stencils and CSE'd algebra for one pointwise spec.
 
     With ``stream_linear``, upwind and KO derivatives are consumed as soon as
@@ -251,7 +251,7 @@ def _ordered_body(spec: specs.Spec, fd_order: int, out_lvalues: List[str], ratio
                 if n in derivs:
                     flops[0] += stencil_flops(derivs[n], fd_order)
                 lines.append(f"{indent}const double {n} = {defs[n]};")
-                if fd.is_deriv_symbol(n):
+                if barriers and fd.is_deriv_symbol(n):
                     lines.append(f"{indent}__builtin_amdgcn_sched_barrier(0);")
 
     outs = [e for _, e in spec.outputs]
@@ -280,15 +280,41 @@ def _ordered_body(spec: specs.Spec, fd_order: int, out_lvalues: List[str], ratio
                 lines.append("  __builtin_amdgcn_sched_barrier(0);")
         outs = cores
     replaced, reduced = sp.cse(outs, symbols=sp.numbered_symbols("t"), order="none")
-    for s, e in replaced:
-        need(e)
-        lines.append(f"  const double {s} = {pr.doprint(e)};")
+    if order == "cse":
+        for s, e in replaced:
+            need(e)
+            lines.append(f"  const double {s} = {pr.doprint(e)};")
+            flops[0] += emit._flops(e)
+    else:
+        # Demand-driven depth-first emission: each output pulls in the temporaries it
+        # needs just before it is stored, so values used by one output die early.
+        tdef = {s: e for s, e in replaced}
+        done: Set = set()
+
+        def emit_t(sym):
+            stack = [(sym, False)]
+            while stack:
+                t, expanded = stack.pop()
+                if t in done:
+                    continue
+                if expanded:
+                    done.add(t)
+                    need(tdef[t])
+                    lines.append(f"  const double {t} = {pr.doprint(tdef[t])};")
+                    flops[0] += emit._flops(tdef[t])
+                    continue
+                stack.append((t, True))
+                for d in sorted((x for x in tdef[t].free_symbols if x in tdef and x not in done), key=str, reverse=True):
+                    stack.append((d, False))
+
     for (g, _), lv, e in zip(spec.outputs, out_lvalues, reduced):
+        if order != "cse":
+            for d in sorted((x for x in e.free_symbols if x in tdef), key=str):
+                emit_t(d)
         need(e)
         tail = f" + {acc_names[g]}" if g in acc_names else ""
         lines.append(f"  {lv} = {pr.doprint(e)}{tail};")
         flops[0] += emit._flops(e) + (1 if tail else 0)
-    flops[0] += sum(emit._flops(e) for _, e in replaced)
     FLOPS[f"{spec.name}_fd{fd_order}"] = flops[0]
     return "\n".join(lines)
 
@@ -388,25 +414,96 @@ RK_EPILOGUE = r"""
 """
 
 
-def bssn_kernels(v: specs.Variant, fd_order: int) -> str:
-    tag = f"{v.tag}_fd{fd_order}"
-    rspec = specs.bssn_rhs_spec(v)
-    rats: Dict[str, str] = {}
-    ngf = len(bssn.EVOL_GFS)
+# Output partitions: each group becomes one kernel with its own CSE, trading
+# recomputation of shared subexpressions for a smaller live register set.
+_A = ["aDD00", "aDD01", "aDD02", "aDD11", "aDD12", "aDD22"]
+_H = ["hDD00", "hDD01", "hDD02", "hDD11", "hDD12", "hDD22"]
+_L = ["lambdaU0", "lambdaU1", "lambdaU2"]
+_B = ["betU0", "betU1", "betU2"]
+_V = ["vetU0", "vetU1", "vetU2"]
+RHS_PARTITIONS = {
+    "p1": [list(bssn.EVOL_GFS)],
+    "p2": [_A + ["alpha", "cf"] + _H + ["trK"], _L + _B + _V],
+    "p3": [_H + ["cf", "alpha"] + _V, _A + ["trK"], _L + _B],
+    "p4": [_H + ["cf", "alpha"] + _V, _A, ["trK"], _L + _B],
+}
+RICCI_PARTITIONS = {
+    "r1": [list(bssn.AUXEVOL_GFS)],
+    "r2": [["RbarDD00", "RbarDD01", "RbarDD02"], ["RbarDD11", "RbarDD12", "RbarDD22"]],
+    "r3": [["RbarDD00", "RbarDD11"], ["RbarDD01", "RbarDD12"], ["RbarDD02", "RbarDD22"]],
+    "r6": [[g] for g in bssn.AUXEVOL_GFS],
+}
+LAUNCH_BOUNDS = {"lb1": "256", "lb2": "256, 2"}
 
-    # ---- plain rhs_eval: interior, writes rhs_gfs
-    lv = [f"rhs[{bssn.EVOL_GFS.index(g)} * Ntot + idx]" for g, _ in rspec.outputs]
-    body = _ordered_body(rspec, fd_order, lv, rats)
-    upw = _upwind_prologue(rspec, fd_order, rats)
-    plain = f"""
-extern "C" __global__ void __launch_bounds__(256)
-bssn_rhs_{tag}(const double *__restrict__ in, const double *__restrict__ aux, double *__restrict__ rhs,
-               const double *__restrict__ rfm0, const double *__restrict__ rfm1, const bhah_kparams p) {{
+# Candidate schedules per build: (kind, partition, launch bounds).
+FULL_CANDIDATES = {
+    "rhs": ["p1_lb1", "p2_lb1", "p3_lb1", "p4_lb1", "p3_lb2", "p4_lb2"],
+    "ricci": ["r1_lb1", "r2_lb1", "r3_lb1", "r6_lb1", "r2_lb2", "r3_lb2"],
+}
+SMALL_CANDIDATES = {"rhs": ["p1_lb1", "p3_lb1"], "ricci": ["r1_lb1", "r2_lb1"]}
+
+CAND_SIG = ("dim3 g, dim3 b, hipStream_t s, const double *in, const double *aux, double *out, "
+            "const double *rfm0, const double *rfm1, bhah_kparams p")
+
+
+def _subspec(spec: specs.Spec, name: str, gfs: List[str]) -> specs.Spec:
+    return specs.Spec(name, [(g, e) for g, e in spec.outputs if g in gfs], spec.gfs, spec.aux, spec.upwind)
+
+
+def candidate_tu(v: specs.Variant, fd_order: int, kind: str, cand: str) -> Tuple[str, int]:
+    """One candidate schedule: group kernels plus a launcher running them in order.
+
+    Returns (source, flops per point summed over the groups)."""
+    tag = f"{v.tag}_fd{fd_order}"
+    part, lb = cand.split("_")
+    rats: Dict[str, str] = {}
+    kernels, calls, total = [], [], 0
+    if kind == "rhs":
+        spec = specs.bssn_rhs_spec(v)
+        groups = RHS_PARTITIONS[part]
+    else:
+        spec = specs.ricci_spec(v)
+        groups = RICCI_PARTITIONS[part]
+    for k, gfs in enumerate(groups):
+        sub = spec if len(groups) == 1 else _subspec(spec, f"{spec.name}_{part}g{k}", gfs)
+        kname = f"bssn_{kind}_{tag}_{cand}_g{k}"
+        if kind == "rhs":
+            lv = [f"out[{bssn.EVOL_GFS.index(g)} * Ntot + idx]" for g, _ in sub.outputs]
+            body = _ordered_body(sub, fd_order, lv, rats)
+            has_up = any(d.op == "dupD" for d in sub.classify()[2])
+            upw = _upwind_prologue(sub, fd_order, rats) if has_up else ""
+        else:
+            lv = [f"out[{bssn.AUXEVOL_GFS.index(g)} * Ntot + idx]" for g, _ in sub.outputs]
+            body = _ordered_body(sub, fd_order, lv, rats)
+            upw = ""
+        total += FLOPS[f"{sub.name}_fd{fd_order}"]
+        kernels.append(f"""
+extern "C" __global__ void __launch_bounds__({LAUNCH_BOUNDS[lb]})
+{kname}(const double *__restrict__ in, const double *__restrict__ aux, double *__restrict__ out,
+        const double *__restrict__ rfm0, const double *__restrict__ rfm1, const bhah_kparams p) {{
 {_prologue(True)}
 {upw}
 {body}
 }}
+""")
+        calls.append(f"  hipLaunchKernelGGL({kname}, g, b, 0, s, in, aux, out, rfm0, rfm1, p);")
+    consts = "\n".join(f"#define {k} ({v})" for k, v in sorted(rats.items()))
+    undefs = "\n".join(f"#undef {k}" for k in sorted(rats))
+    launcher = f"""
+extern "C" hipError_t launch_bssn_{kind}_{tag}_{cand}({CAND_SIG}) {{
+{chr(10).join(calls)}
+  return hipGetLastError();
+}}
 """
+    head = f"// ---- {kind} candidate {cand}: {len(groups)} kernel(s); {v.coord}, fd_order={fd_order}\n"
+    return head + consts + "\n" + "".join(kernels) + undefs + "\n" + launcher, total
+
+
+def bssn_kernels(v: specs.Variant, fd_order: int) -> str:
+    """Kernels shared by every schedule: RK4+det pass, det enforcement, constraints."""
+    tag = f"{v.tag}_fd{fd_order}"
+    rats: Dict[str, str] = {}
+    ngf = len(bssn.EVOL_GFS)
     # ---- RK4 substep update + det(gammabar) enforcement, all points, one streaming pass
     det = detgamma_inline(v, rats, "nxt")
     rkdet = f"""
@@ -427,18 +524,6 @@ bssn_rk_det_{tag}(const double *__restrict__ kin, const double *y_n, double *__r
   for (int g = 0; g < NGF; g++) out[g * Ntot + idx] = nxt[g];
 }}
 """
-    # ---- Ricci
-    cspec = specs.ricci_spec(v)
-    lv = [f"aux[{bssn.AUXEVOL_GFS.index(g)} * Ntot + idx]" for g, _ in cspec.outputs]
-    body = _ordered_body(cspec, fd_order, lv, rats)
-    ricci = f"""
-extern "C" __global__ void __launch_bounds__(256)
-bssn_ricci_{tag}(const double *__restrict__ in, double *__restrict__ aux,
-                 const double *__restrict__ rfm0, const double *__restrict__ rfm1, const bhah_kparams p) {{
-{_prologue(True)}
-{body}
-}}
-"""
     # ---- det enforcement over all points (standalone ABI call)
     dspec = specs.detgamma_spec(v)
     det_all = f"""
@@ -454,7 +539,7 @@ bssn_detgamma_{tag}(double *__restrict__ gfs, const double *__restrict__ rfm0, c
 {"".join(f"  gfs[{bssn.EVOL_GFS.index(g)} * Ntot + idx] = nxt[{bssn.EVOL_GFS.index(g)}];{chr(10)}" for g, _ in dspec.outputs)}
 }}
 """
-    # ---- constraints H, MU^i at interior points
+    # ---- constraints H, MU^i, M^2 at interior points
     kspec = specs.constraints_spec(v)
     lv = [f"diag[{k} * Ntot + idx]" for k in range(len(kspec.outputs))]
     body = _ordered_body(kspec, fd_order, lv, rats)
@@ -471,11 +556,6 @@ bssn_constraints_{tag}(const double *__restrict__ in, const double *__restrict__
     undefs = "\n".join(f"#undef {k}" for k in sorted(rats))
     launch = f"""
 // Host-side launchers (one translation unit per variant; no relocatable device code needed).
-extern "C" hipError_t launch_bssn_rhs_{tag}(dim3 g, dim3 b, hipStream_t s, const double *in, const double *aux,
-    double *rhs, const double *rfm0, const double *rfm1, bhah_kparams p) {{
-  hipLaunchKernelGGL(bssn_rhs_{tag}, g, b, 0, s, in, aux, rhs, rfm0, rfm1, p);
-  return hipGetLastError();
-}}
 extern "C" hipError_t launch_bssn_rk_det_{tag}(dim3 g, dim3 b, hipStream_t s, const double *kin, const double *y_n,
     double *y_run, double *out, const double *rfm0, const double *rfm1, bhah_kparams p) {{
   hipLaunchKernelGGL(bssn_rk_det_{tag}, g, b, 0, s, kin, y_n, y_run, out, rfm0, rfm1, p);
@@ -486,18 +566,13 @@ extern "C" hipError_t launch_bssn_constraints_{tag}(dim3 g, dim3 b, hipStream_t 
   hipLaunchKernelGGL(bssn_constraints_{tag}, g, b, 0, s, in, aux, diag, rfm0, rfm1, p);
   return hipGetLastError();
 }}
-extern "C" hipError_t launch_bssn_ricci_{tag}(dim3 g, dim3 b, hipStream_t s, const double *in, double *aux,
-    const double *rfm0, const double *rfm1, bhah_kparams p) {{
-  hipLaunchKernelGGL(bssn_ricci_{tag}, g, b, 0, s, in, aux, rfm0, rfm1, p);
-  return hipGetLastError();
-}}
 extern "C" hipError_t launch_bssn_detgamma_{tag}(dim3 g, dim3 b, hipStream_t s, double *gfs, const double *rfm0,
     const double *rfm1, bhah_kparams p) {{
   hipLaunchKernelGGL(bssn_detgamma_{tag}, g, b, 0, s, gfs, rfm0, rfm1, p);
   return hipGetLastError();
 }}
 """
-    return "\n".join(head) + "\n" + consts + "\n" + plain + rkdet + ricci + det_all + cons + undefs + "\n" + launch
+    return "\n".join(head) + "\n" + consts + "\n" + rkdet + det_all + cons + undefs + "\n" + launch
 
 
 def wave_kernels(fd_order: int) -> str:
@@ -547,11 +622,12 @@ extern "C" hipError_t launch_wave_rhs_rk_{tag}(dim3 g, dim3 b, hipStream_t s, co
 
 
 BSSN_BUILDS = [
-    (specs.VARIANTS[0], 4),
-    (specs.VARIANTS[1], 6),
-    (specs.VARIANTS[1], 4),
+    (specs.VARIANTS[0], 4, SMALL_CANDIDATES),
+    (specs.VARIANTS[1], 6, FULL_CANDIDATES),
+    (specs.VARIANTS[1], 4, SMALL_CANDIDATES),
 ]
 WAVE_BUILDS = [4]
+COORD_ENUM = {"Cartesian": 0, "Spherical": 1, "SinhSpherical": 2}
 
 
 def kernel_name_tag(v: specs.Variant, fd_order: int) -> str:
@@ -559,23 +635,66 @@ def kernel_name_tag(v: specs.Variant, fd_order: int) -> str:
 
 
 def generate(outdir: str) -> List[str]:
+    import json
+
     os.makedirs(outdir, exist_ok=True)
+    for f in os.listdir(outdir):
+        if f.endswith(".hip") or f.endswith(".inc"):
+            os.remove(os.path.join(outdir, f))
     paths = []
     hdr = "// Generated by nrpy_amd/codegen/hip_gen.py -- do not edit.\n#include <hip/hip_runtime.h>\n#include <math.h>\n" + KPARAMS
-    for v, order in BSSN_BUILDS:
-        path = os.path.join(outdir, f"bssn_{kernel_name_tag(v, order)}.hip")
+    meta = {"flops_per_point": {}, "candidates": {}}
+    table_decl, table_rows = [], []
+    for v, order, cands in BSSN_BUILDS:
+        tag = kernel_name_tag(v, order)
+        path = os.path.join(outdir, f"bssn_{tag}.hip")
         with open(path, "w", encoding="utf-8") as f:
             f.write(hdr + bssn_kernels(v, order))
         paths.append(path)
+        arrays = {}
+        for kind in ("rhs", "ricci"):
+            rows = []
+            for cand in cands[kind]:
+                src, fl = candidate_tu(v, order, kind, cand)
+                path = os.path.join(outdir, f"bssn_{tag}_{kind}_{cand}.hip")
+                with open(path, "w", encoding="utf-8") as f:
+                    f.write(hdr + src)
+                paths.append(path)
+                meta["candidates"][f"{kind}_{tag}_{cand}"] = fl
+                table_decl.append(f'extern "C" hipError_t launch_bssn_{kind}_{tag}_{cand}({CAND_SIG});')
+                rows.append(f'{{"{cand}", launch_bssn_{kind}_{tag}_{cand}}}')
+            arrays[kind] = f"kCand_{kind}_{tag}"
+            table_decl.append(f"static const BssnCandidate {arrays[kind]}[] = {{{', '.join(rows)}}};")
+        sym_mask = sum(1 << int(c) for c in v.symmetry_axes)
+        table_rows.append(
+            f"    {{{COORD_ENUM[v.coord]}, {sym_mask}, {int(v.enable_KO)}, {order}, \"{tag}\", "
+            f"{arrays['rhs']}, {len(cands['rhs'])}, {arrays['ricci']}, {len(cands['ricci'])}, "
+            f"launch_bssn_rk_det_{tag}, launch_bssn_detgamma_{tag}, launch_bssn_constraints_{tag}}},"
+        )
+        table_decl.append(
+            f'extern "C" hipError_t launch_bssn_rk_det_{tag}(dim3, dim3, hipStream_t, const double *, const double *, '
+            f"double *, double *, const double *, const double *, bhah_kparams);"
+        )
+        table_decl.append(
+            f'extern "C" hipError_t launch_bssn_detgamma_{tag}(dim3, dim3, hipStream_t, double *, const double *, '
+            f"const double *, bhah_kparams);"
+        )
+        table_decl.append(
+            f'extern "C" hipError_t launch_bssn_constraints_{tag}(dim3, dim3, hipStream_t, const double *, '
+            f"const double *, double *, const double *, const double *, bhah_kparams);"
+        )
     for order in WAVE_BUILDS:
         path = os.path.join(outdir, f"wave_fd{order}.hip")
         with open(path, "w", encoding="utf-8") as f:
             f.write(hdr + wave_kernels(order))
         paths.append(path)
-    import json
-
+    with open(os.path.join(outdir, "kernel_table.inc"), "w", encoding="utf-8") as f:
+        f.write("// Generated by nrpy_amd/codegen/hip_gen.py -- do not edit.\n")
+        f.write("\n".join(table_decl) + "\n")
+        f.write("static const BssnKernels kBssnKernels[] = {\n" + "\n".join(table_rows) + "\n};\n")
+    meta["flops_per_point"] = dict(sorted(FLOPS.items()))
     with open(os.path.join(outdir, "kernel_meta.json"), "w", encoding="utf-8") as f:
-        json.dump({"flops_per_point": dict(sorted(FLOPS.items()))}, f, indent=1)
+        json.dump(meta, f, indent=1, sort_keys=True)
     return paths
 
 

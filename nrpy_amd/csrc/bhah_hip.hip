@@ -26,47 +26,35 @@ struct bhah_kparams {
 };
 
 // ---------------------------------------------------------------------------
-// Generated kernel launchers, one set per (coordinate, symmetry, KO, fd order).
-#define BHAH_DECL_BSSN(tag)                                                                                           \
-  extern "C" hipError_t launch_bssn_rhs_##tag(dim3, dim3, hipStream_t, const double *, const double *, double *,      \
-                                              const double *, const double *, bhah_kparams);                           \
-  extern "C" hipError_t launch_bssn_rk_det_##tag(dim3, dim3, hipStream_t, const double *, const double *, double *,   \
-                                                 double *, const double *, const double *, bhah_kparams);              \
-  extern "C" hipError_t launch_bssn_ricci_##tag(dim3, dim3, hipStream_t, const double *, double *, const double *,    \
-                                                const double *, bhah_kparams);                                         \
-  extern "C" hipError_t launch_bssn_detgamma_##tag(dim3, dim3, hipStream_t, double *, const double *, const double *, \
-                                                   bhah_kparams);                                                      \
-  extern "C" hipError_t launch_bssn_constraints_##tag(dim3, dim3, hipStream_t, const double *, const double *,        \
-                                                      double *, const double *, const double *, bhah_kparams);
-BHAH_DECL_BSSN(Spherical_sym2_KO0_fd4)
-BHAH_DECL_BSSN(SinhSpherical_symnone_KO1_fd6)
-BHAH_DECL_BSSN(SinhSpherical_symnone_KO1_fd4)
+// Generated kernels: per (coordinate, symmetry, KO, fd order) build, a set of
+// candidate schedules for rhs_eval and Ricci_eval (output partitions x register
+// budgets), plus the RK4+det pass, det enforcement and constraints.
+typedef hipError_t (*bhah_cand_fn)(dim3, dim3, hipStream_t, const double *, const double *, double *,
+                                   const double *, const double *, bhah_kparams);
+struct BssnCandidate {
+  const char *name;
+  bhah_cand_fn launch;
+};
+struct BssnKernels {
+  int coord, sym_axes, KO, fd_order;
+  const char *tag;
+  const BssnCandidate *rhs;
+  int n_rhs;
+  const BssnCandidate *ricci;
+  int n_ricci;
+  hipError_t (*rk_det)(dim3, dim3, hipStream_t, const double *, const double *, double *, double *, const double *,
+                       const double *, bhah_kparams);
+  hipError_t (*detgamma)(dim3, dim3, hipStream_t, double *, const double *, const double *, bhah_kparams);
+  hipError_t (*constraints)(dim3, dim3, hipStream_t, const double *, const double *, double *, const double *,
+                            const double *, bhah_kparams);
+};
+#include "gen/kernel_table.inc"
+
 extern "C" hipError_t launch_wave_rhs_fd4(dim3, dim3, hipStream_t, const double *, double *, bhah_kparams);
 extern "C" hipError_t launch_wave_rhs_rk_fd4(dim3, dim3, hipStream_t, const double *, const double *, double *,
                                              double *, bhah_kparams);
 
 namespace {
-
-struct BssnKernels {
-  int coord, sym_axes, KO, fd_order;
-  hipError_t (*rhs)(dim3, dim3, hipStream_t, const double *, const double *, double *, const double *, const double *,
-                    bhah_kparams);
-  hipError_t (*rk_det)(dim3, dim3, hipStream_t, const double *, const double *, double *, double *, const double *,
-                       const double *, bhah_kparams);
-  hipError_t (*ricci)(dim3, dim3, hipStream_t, const double *, double *, const double *, const double *, bhah_kparams);
-  hipError_t (*detgamma)(dim3, dim3, hipStream_t, double *, const double *, const double *, bhah_kparams);
-  hipError_t (*constraints)(dim3, dim3, hipStream_t, const double *, const double *, double *, const double *,
-                            const double *, bhah_kparams);
-};
-
-#define BHAH_KSET(coord, sym, ko, fd, tag)                                                                            \
-  {coord, sym, ko, fd, launch_bssn_rhs_##tag, launch_bssn_rk_det_##tag, launch_bssn_ricci_##tag,                   \
-   launch_bssn_detgamma_##tag, launch_bssn_constraints_##tag}
-const BssnKernels kBssnKernels[] = {
-    BHAH_KSET(BHAH_SPHERICAL, 4, 0, 4, Spherical_sym2_KO0_fd4),
-    BHAH_KSET(BHAH_SINHSPHERICAL, 0, 1, 6, SinhSpherical_symnone_KO1_fd6),
-    BHAH_KSET(BHAH_SINHSPHERICAL, 0, 1, 4, SinhSpherical_symnone_KO1_fd4),
-};
 
 thread_local std::string g_last_error;
 
@@ -115,6 +103,9 @@ struct bhah_patch {
   int *d_gf_parity = nullptr;
   hipStream_t stream = nullptr;
   bool own_stream = false;
+  // selected schedules (indices into k->rhs / k->ricci) and interior tile shape
+  int rhs_choice = 0, ricci_choice = 0;
+  int tile_x = 64, tile_y = 4;
   // cached graph for MoL_step_n
   hipGraphExec_t graph_exec = nullptr;
   bhah_MoL_gridfunctions graph_gf{};
@@ -140,10 +131,16 @@ struct bhah_patch {
     p.substep = substep;
     return p;
   }
-  // Interior launch: 64 x 4 threads over (i0, i1), one block row per i2.
+  // Interior launch: tile_x x tile_y (= 256) threads over (i0, i1), one block row per i2.
   void interior_dims(dim3 &g, dim3 &b) const {
-    b = dim3(64, 4, 1);
-    g = dim3((Nx[0] - 2 * NG + 63) / 64, (Nx[1] - 2 * NG + 3) / 4, Nx[2] - 2 * NG);
+    b = dim3(tile_x, tile_y, 1);
+    g = dim3((Nx[0] - 2 * NG + tile_x - 1) / tile_x, (Nx[1] - 2 * NG + tile_y - 1) / tile_y, Nx[2] - 2 * NG);
+  }
+  void invalidate_graph() {
+    if (graph_exec) {
+      hipGraphExecDestroy(graph_exec);
+      graph_exec = nullptr;
+    }
   }
   void all_dims(dim3 &g, dim3 &b) const {
     b = dim3(64, 4, 1);
@@ -708,7 +705,7 @@ int bhah_hip_rhs_eval(bhah_patch *P, const double *aux, const double *in, double
   if ((rc = check_ptr(aux, "auxevol_gfs")) || (rc = check_ptr(in, "in_gfs")) || (rc = check_ptr(rhs, "rhs_gfs"))) return rc;
   dim3 g, b;
   P->interior_dims(g, b);
-  HIP_TRY(P->k->rhs(g, b, P->stream, in, aux, rhs, P->d_rfm0, P->d_rfm1, P->kp(0, 0)));
+  HIP_TRY(P->k->rhs[P->rhs_choice].launch(g, b, P->stream, in, aux, rhs, P->d_rfm0, P->d_rfm1, P->kp(0, 0)));
   return BHAH_OK;
 }
 
@@ -718,7 +715,7 @@ int bhah_hip_Ricci_eval(bhah_patch *P, const double *in, double *aux) {
   if ((rc = check_ptr(in, "in_gfs")) || (rc = check_ptr(aux, "auxevol_gfs"))) return rc;
   dim3 g, b;
   P->interior_dims(g, b);
-  HIP_TRY(P->k->ricci(g, b, P->stream, in, aux, P->d_rfm0, P->d_rfm1, P->kp(0, 0)));
+  HIP_TRY(P->k->ricci[P->ricci_choice].launch(g, b, P->stream, in, nullptr, aux, P->d_rfm0, P->d_rfm1, P->kp(0, 0)));
   return BHAH_OK;
 }
 
@@ -894,6 +891,44 @@ int bhah_hip_MoL_step_n(bhah_patch *P, bhah_MoL_gridfunctions *gf, double dt, in
     P->graph_dt = dt;
   }
   for (int n = 0; n < nsteps; n++) HIP_TRY(hipGraphLaunch(P->graph_exec, P->stream));
+  return BHAH_OK;
+}
+
+int bhah_hip_patch_num_schedules(const bhah_patch *P, int kind) {
+  if (!P || !P->k) return fail(BHAH_ERR_ARG, "patch is not a BSSN patch");
+  if (kind == BHAH_KERNEL_RHS) return P->k->n_rhs;
+  if (kind == BHAH_KERNEL_RICCI) return P->k->n_ricci;
+  return fail(BHAH_ERR_ARG, "unknown kernel kind %d", kind);
+}
+
+const char *bhah_hip_patch_schedule_name(const bhah_patch *P, int kind, int index) {
+  if (!P || !P->k) return nullptr;
+  if (kind == BHAH_KERNEL_RHS && index >= 0 && index < P->k->n_rhs) return P->k->rhs[index].name;
+  if (kind == BHAH_KERNEL_RICCI && index >= 0 && index < P->k->n_ricci) return P->k->ricci[index].name;
+  return nullptr;
+}
+
+int bhah_hip_patch_select_schedule(bhah_patch *P, int kind, int index) {
+  const int n = bhah_hip_patch_num_schedules(P, kind);
+  if (n < 0) return n;
+  if (index < 0 || index >= n) return fail(BHAH_ERR_ARG, "schedule index %d out of range [0,%d)", index, n);
+  if (kind == BHAH_KERNEL_RHS) P->rhs_choice = index;
+  else P->ricci_choice = index;
+  P->invalidate_graph();
+  return BHAH_OK;
+}
+
+int bhah_hip_patch_get_schedule(const bhah_patch *P, int kind) {
+  if (!P || !P->k) return fail(BHAH_ERR_ARG, "patch is not a BSSN patch");
+  return kind == BHAH_KERNEL_RHS ? P->rhs_choice : P->ricci_choice;
+}
+
+int bhah_hip_patch_set_tile(bhah_patch *P, int tile_x, int tile_y) {
+  if (!P) return fail(BHAH_ERR_ARG, "NULL patch");
+  if (tile_x * tile_y != 256 || tile_x < 16 || tile_x % 16) return fail(BHAH_ERR_ARG, "tile must be 256 threads, tile_x a multiple of 16");
+  P->tile_x = tile_x;
+  P->tile_y = tile_y;
+  P->invalidate_graph();
   return BHAH_OK;
 }
 

@@ -24,6 +24,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libbhah_hip.so")
 CARTESIAN, SPHERICAL, SINHSPHERICAL = 0, 1, 2
 SYSTEM_BSSN, SYSTEM_WAVE = 0, 1
 OUTER_BC_RADIATION, OUTER_BC_EXTRAPOLATION = 0, 1
+KERNEL_RHS, KERNEL_RICCI = 0, 1
 
 EVOL_GFS = [
     "aDD00", "aDD01", "aDD02", "aDD11", "aDD12", "aDD22", "alpha",
@@ -107,6 +108,11 @@ SIGNATURES: Dict[str, tuple] = {
     "bhah_hip_MoL_free": (_I, [_P, ctypes.POINTER(MoLGridfunctions)]),
     "bhah_hip_MoL_step_forward_in_time": (_I, [_P, ctypes.POINTER(MoLGridfunctions), _D, ctypes.POINTER(_D)]),
     "bhah_hip_MoL_step_n": (_I, [_P, ctypes.POINTER(MoLGridfunctions), _D, _I, _I]),
+    "bhah_hip_patch_num_schedules": (_I, [_P, _I]),
+    "bhah_hip_patch_schedule_name": (ctypes.c_char_p, [_P, _I, _I]),
+    "bhah_hip_patch_select_schedule": (_I, [_P, _I, _I]),
+    "bhah_hip_patch_get_schedule": (_I, [_P, _I]),
+    "bhah_hip_patch_set_tile": (_I, [_P, _I, _I]),
     "bhah_hip_wave_rhs_eval": (_I, [_P, _P, _P]),
     "bhah_hip_wave_apply_bcs": (_I, [_P, _P]),
     "bhah_hip_diagnostics_constraints": (_I, [_P, _P, _P, _P, ctypes.POINTER(_D)]),
@@ -194,7 +200,7 @@ def _ptr(t) -> int:
 class Patch:
     """One grid patch: geometry, boundary structs and kernels on the current device."""
 
-    def __init__(self, cfg: PatchConfig, stream=None):
+    def __init__(self, cfg: PatchConfig, stream=None, tuned: bool = True):
         import torch
 
         self.lib = load_library()
@@ -215,6 +221,12 @@ class Patch:
         self.dxx = tuple(info.dxx)
         s = stream if stream is not None else torch.cuda.current_stream()
         _check(self.lib.bhah_hip_patch_set_stream(self.handle, ctypes.c_void_p(s.cuda_stream)))
+        self.tile = (64, 4)
+        self.tuned = None
+        if tuned:
+            from .autotune import apply_tuned
+
+            self.tuned = apply_tuned(self)
 
     def close(self) -> None:
         if getattr(self, "handle", None):
@@ -261,6 +273,25 @@ class Patch:
         if n:
             _check(self.lib.bhah_hip_patch_get_outer_bc(self.handle, gz, dirn, out.ctypes.data))
         return out
+
+    # ---- kernel schedules (autotuned)
+    def schedules(self, kind: int):
+        n = self.lib.bhah_hip_patch_num_schedules(self.handle, kind)
+        if n < 0:
+            _check(n)
+        return [self.lib.bhah_hip_patch_schedule_name(self.handle, kind, i).decode() for i in range(n)]
+
+    def select_schedule(self, kind: int, name_or_index):
+        names = self.schedules(kind)
+        idx = names.index(name_or_index) if isinstance(name_or_index, str) else int(name_or_index)
+        _check(self.lib.bhah_hip_patch_select_schedule(self.handle, kind, idx))
+
+    def schedule(self, kind: int) -> str:
+        return self.schedules(kind)[self.lib.bhah_hip_patch_get_schedule(self.handle, kind)]
+
+    def set_tile(self, tile_x: int, tile_y: int):
+        _check(self.lib.bhah_hip_patch_set_tile(self.handle, tile_x, tile_y))
+        self.tile = (tile_x, tile_y)
 
     # ---- reference API counterparts
     def rhs_eval(self, auxevol, in_gfs, rhs):

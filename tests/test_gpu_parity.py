@@ -182,3 +182,32 @@ def test_wave_matches_oracle_and_exact(hip, oracle_lib):
         S.step(dt)
     er = rel_err(S.y_n.cpu().numpy(), st["y_n"])
     assert er.max() < 1e-10, er
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_every_schedule_and_tile_matches_oracle(hip, oracle_lib, case):
+    """All generated rhs/Ricci schedules and tile shapes compute the same values."""
+    import torch
+    from nrpy_amd.runtime import KERNEL_RHS, KERNEL_RICCI
+
+    P, O = make_pair(hip, oracle_lib, case)
+    y = state(O, 3)
+    aux_o = O.ricci(y)
+    rhs_o = O.rhs(y, aux_o, O.params())
+    yd = torch.from_numpy(y).cuda()
+    auxd = torch.from_numpy(aux_o).cuda()
+    NG = O.NG
+    for tile in [(64, 4), (128, 2), (32, 8)]:
+        P.set_tile(*tile)
+        for name in P.schedules(KERNEL_RICCI):
+            P.select_schedule(KERNEL_RICCI, name)
+            a = P.new_gfs(6)
+            P.Ricci_eval(yd, a)
+            er = rel_err(interior(a.cpu().numpy(), NG), interior(aux_o, NG))
+            assert er.max() < 1e-12, (tile, name, er)
+        for name in P.schedules(KERNEL_RHS):
+            P.select_schedule(KERNEL_RHS, name)
+            r = P.new_gfs()
+            P.rhs_eval(auxd, yd, r)
+            er = rel_err(interior(r.cpu().numpy(), NG), interior(rhs_o, NG))
+            assert er.max() < 1e-12, (tile, name, er)
