@@ -227,7 +227,8 @@ def _emit_defs(spec: specs.Spec, fd_order: int):
 
 
 def _ordered_body(spec: specs.Spec, fd_order: int, out_lvalues: List[str], rationals: Dict[str, str],
-                  stream_linear: bool = True, order: str = "dfs", barriers: bool = True) -> str:
+                  stream_linear: bool = True, order: str = "dfs", barriers: bool = True,
+                  cse_scope: str = "joint") -> str:
     """Stencils and CSE'd algebra for one pointwise spec.
 
     With ``stream_linear``, upwind and KO derivatives are consumed as soon as
@@ -279,7 +280,31 @@ def _ordered_body(spec: specs.Spec, fd_order: int, out_lvalues: List[str], ratio
                     flops[0] += 2 + stencil_flops(d, fd_order)
                 lines.append("  __builtin_amdgcn_sched_barrier(0);")
         outs = cores
-    replaced, reduced = sp.cse(outs, symbols=sp.numbered_symbols("t"), order="none")
+    sdefs = spec.used_defs()
+    if sdefs and cse_scope == "local":
+        # CSE within each staged definition and each output separately: temporaries
+        # then live only inside one stage, at the cost of recomputing cheap
+        # reference-metric products shared between stages.
+        tsyms = sp.numbered_symbols("t")
+        replaced, reduced_defs = [], []
+        for s_, e_ in sdefs:
+            r_, d_ = sp.cse([e_], symbols=tsyms, order="none")
+            replaced += r_
+            reduced_defs.append((s_, d_[0]))
+        reduced = []
+        for e_ in outs:
+            r_, d_ = sp.cse([e_], symbols=tsyms, order="none")
+            replaced += r_
+            reduced.append(d_[0])
+        replaced += reduced_defs
+        order = "dfs"
+    else:
+        replaced, reduced = sp.cse([e for _, e in sdefs] + list(outs), symbols=sp.numbered_symbols("t"), order="none")
+        staged_defs = [(s_, e_) for (s_, _), e_ in zip(sdefs, reduced[: len(sdefs)])]
+        reduced = reduced[len(sdefs):]
+        if staged_defs:
+            replaced = list(replaced) + staged_defs
+            order = "dfs"
     if order == "cse":
         for s, e in replaced:
             need(e)
@@ -437,44 +462,56 @@ LAUNCH_BOUNDS = {"lb1": "256", "lb2": "256, 2"}
 
 # Candidate schedules per build: (kind, partition, launch bounds).
 FULL_CANDIDATES = {
-    "rhs": ["p1_lb1", "p2_lb1", "p3_lb1", "p4_lb1", "p3_lb2", "p4_lb2"],
-    "ricci": ["r1_lb1", "r2_lb1", "r3_lb1", "r6_lb1", "r2_lb2", "r3_lb2"],
+    "rhs": ["p1_lb1", "p1_lb1_sj", "p1_lb1_sl", "p1_lb2_sl", "p2_lb1_sl", "p3_lb1", "p3_lb1_sl", "p3_lb2_sl",
+            "p4_lb1"],
+    "ricci": ["r1_lb1", "r1_lb1_sj", "r1_lb1_sl", "r1_lb2_sl", "r2_lb1", "r2_lb1_sl", "r2_lb2_sl", "r3_lb1",
+              "r3_lb2_sl", "r6_lb1"],
 }
-SMALL_CANDIDATES = {"rhs": ["p1_lb1", "p3_lb1"], "ricci": ["r1_lb1", "r2_lb1"]}
+SMALL_CANDIDATES = {"rhs": ["p1_lb1", "p1_lb1_sl", "p3_lb1_sl"], "ricci": ["r1_lb1", "r1_lb1_sl", "r2_lb1_sl"]}
 
 CAND_SIG = ("dim3 g, dim3 b, hipStream_t s, const double *in, const double *aux, double *out, "
             "const double *rfm0, const double *rfm1, bhah_kparams p")
 
 
 def _subspec(spec: specs.Spec, name: str, gfs: List[str]) -> specs.Spec:
-    return specs.Spec(name, [(g, e) for g, e in spec.outputs if g in gfs], spec.gfs, spec.aux, spec.upwind)
+    return specs.Spec(name, [(g, e) for g, e in spec.outputs if g in gfs], spec.gfs, spec.aux, spec.upwind,
+                      defs=spec.defs)
 
 
 def candidate_tu(v: specs.Variant, fd_order: int, kind: str, cand: str) -> Tuple[str, int]:
     """One candidate schedule: group kernels plus a launcher running them in order.
 
+    ``cand`` is ``<partition>_<launch bounds>[_<mode>]``; mode ``x`` (default) CSEs the
+    fully expanded expressions, ``sj`` evaluates named intermediate tensors with one
+    joint CSE, ``sl`` with a separate CSE per intermediate (shortest live ranges).
     Returns (source, flops per point summed over the groups)."""
     tag = f"{v.tag}_fd{fd_order}"
-    part, lb = cand.split("_")
+    parts = cand.split("_")
+    part, lb = parts[0], parts[1]
+    mode = parts[2] if len(parts) > 2 else "x"
+    staged = mode in ("sj", "sl")
+    scope = "local" if mode == "sl" else "joint"
     rats: Dict[str, str] = {}
     kernels, calls, total = [], [], 0
     if kind == "rhs":
-        spec = specs.bssn_rhs_spec(v)
+        spec = specs.bssn_rhs_spec(v, staged=staged)
         groups = RHS_PARTITIONS[part]
     else:
-        spec = specs.ricci_spec(v)
+        spec = specs.ricci_spec(v, staged=staged)
         groups = RICCI_PARTITIONS[part]
     for k, gfs in enumerate(groups):
         sub = spec if len(groups) == 1 else _subspec(spec, f"{spec.name}_{part}g{k}", gfs)
+        if scope == "local":
+            sub = _subspec(sub, f"{sub.name}_local", [g for g, _ in sub.outputs])
         kname = f"bssn_{kind}_{tag}_{cand}_g{k}"
         if kind == "rhs":
             lv = [f"out[{bssn.EVOL_GFS.index(g)} * Ntot + idx]" for g, _ in sub.outputs]
-            body = _ordered_body(sub, fd_order, lv, rats)
+            body = _ordered_body(sub, fd_order, lv, rats, cse_scope=scope)
             has_up = any(d.op == "dupD" for d in sub.classify()[2])
             upw = _upwind_prologue(sub, fd_order, rats) if has_up else ""
         else:
             lv = [f"out[{bssn.AUXEVOL_GFS.index(g)} * Ntot + idx]" for g, _ in sub.outputs]
-            body = _ordered_body(sub, fd_order, lv, rats)
+            body = _ordered_body(sub, fd_order, lv, rats, cse_scope=scope)
             upw = ""
         total += FLOPS[f"{sub.name}_fd{fd_order}"]
         kernels.append(f"""

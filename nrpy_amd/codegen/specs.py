@@ -61,15 +61,33 @@ class Spec:
     gfs: List[str]  # evolved gridfunction order (array layout)
     aux: List[str] = field(default_factory=list)
     upwind: List[sp.Expr] | None = None  # control vector betaU[d]
+    # staged intermediates (symbol, definition), in dependency order
+    defs: List[Tuple[sp.Symbol, sp.Expr]] = field(default_factory=list)
 
     def symbols(self) -> List[str]:
         names = set()
         for _, e in self.outputs:
             names |= {str(s) for s in e.free_symbols}
+        for _, e in self.defs:
+            names |= {str(s) for s in e.free_symbols}
         if self.upwind is not None:
             for e in self.upwind:
                 names |= {str(s) for s in sp.sympify(e).free_symbols}
-        return sorted(names)
+        staged = {str(s) for s, _ in self.defs}
+        return sorted(names - staged)
+
+    def used_defs(self) -> List[Tuple[sp.Symbol, sp.Expr]]:
+        """The staged definitions the outputs actually need (transitively)."""
+        dmap = dict(self.defs)
+        need = set()
+        stack = [s for _, e in self.outputs for s in e.free_symbols if s in dmap]
+        while stack:
+            s = stack.pop()
+            if s in need:
+                continue
+            need.add(s)
+            stack.extend(x for x in dmap[s].free_symbols if x in dmap and x not in need)
+        return [(s, e) for s, e in self.defs if s in need]
 
     def classify(self):
         """Split symbols into (centre gfs, aux gfs, derivs, rfm, params)."""
@@ -100,10 +118,11 @@ def _with_axes(axes: str):
 
 
 @functools.lru_cache(maxsize=None)
-def bssn_rhs_spec(v: Variant) -> Spec:
+def bssn_rhs_spec(v: Variant, staged: bool = False) -> Spec:
     old = _with_axes(v.symmetry_axes)
+    stager = bssn.Stager() if staged else None
     try:
-        ex = bssn.bssn_rhs_exprs(v.coord, ricci_symbolic=True, enable_KO=v.enable_KO)
+        ex = bssn.bssn_rhs_exprs(v.coord, ricci_symbolic=True, enable_KO=v.enable_KO, stager=stager)
         rfm = reference_metric(v.coord)
         vet = tensors.declare("vetU", 1)
         upwind = [vet[i] * rfm.ReU[i] for i in range(3)]
@@ -111,7 +130,9 @@ def bssn_rhs_spec(v: Variant) -> Spec:
         tensors.set_symmetry_axes(old)
     outs = [(_rhs_key_gf(k), e) for k, e in ex.items()]
     outs.sort(key=lambda t: bssn.EVOL_GFS.index(t[0]))
-    return Spec(f"bssn_rhs_{v.tag}", outs, list(bssn.EVOL_GFS), list(bssn.AUXEVOL_GFS), upwind)
+    name = f"bssn_rhs_{v.tag}" + ("_staged" if staged else "")
+    return Spec(name, outs, list(bssn.EVOL_GFS), list(bssn.AUXEVOL_GFS), upwind,
+                defs=stager.defs if stager else [])
 
 
 def _rhs_key_gf(key: str) -> str:
@@ -122,13 +143,15 @@ def _rhs_key_gf(key: str) -> str:
 
 
 @functools.lru_cache(maxsize=None)
-def ricci_spec(v: Variant) -> Spec:
+def ricci_spec(v: Variant, staged: bool = False) -> Spec:
     old = _with_axes(v.symmetry_axes)
+    stager = bssn.Stager() if staged else None
     try:
-        ex = bssn.ricci_exprs(v.coord)
+        ex = bssn.ricci_exprs(v.coord, stager=stager)
     finally:
         tensors.set_symmetry_axes(old)
-    return Spec(f"ricci_{v.tag}", list(ex.items()), list(bssn.EVOL_GFS), [])
+    name = f"ricci_{v.tag}" + ("_staged" if staged else "")
+    return Spec(name, list(ex.items()), list(bssn.EVOL_GFS), [], defs=stager.defs if stager else [])
 
 
 @functools.lru_cache(maxsize=None)

@@ -45,6 +45,44 @@ F_INFINITY["cf"] = 1.0
 F_INFINITY["alpha"] = 1.0
 
 
+class Stager:
+    """Name intermediate tensors so generated code evaluates them once, in stages.
+
+    ``st(name, tensor)`` returns the tensor with each distinct non-trivial entry
+    replaced by a symbol ``_<name><i..>`` and records (symbol, definition) in
+    ``defs``.  Entries holding upwinded or Kreiss-Oliger derivatives stay inline, so
+    those still enter the right-hand sides linearly.  ``expand(e)`` substitutes the
+    definitions back (for checking against the unstaged expressions).
+    """
+
+    def __init__(self) -> None:
+        self.defs: List = []
+        self._by_expr: Dict = {}
+
+    def __call__(self, name: str, t, _idx: str = ""):
+        if isinstance(t, list):
+            return [self(name, x, _idx + str(i)) for i, x in enumerate(t)]
+        t = sp.sympify(t)
+        if t.is_Atom or any(("_dupD" in str(x) or "_dKOD" in str(x)) for x in t.free_symbols):
+            return t
+        key = sp.srepr(t)
+        if key not in self._by_expr:
+            sym = sp.Symbol(f"_{name}{_idx}", real=True)
+            self._by_expr[key] = sym
+            self.defs.append((sym, t))
+        return self._by_expr[key]
+
+    def expand(self, e):
+        sub = {}
+        for sym, d in self.defs:
+            sub[sym] = d.xreplace(sub)
+        return sp.sympify(e).xreplace(sub)
+
+
+def _nostage(name, t):
+    return t
+
+
 def parity_type(gf: str) -> int:
     """Inner-BC parity type of a gridfunction (nrpy/grid.py:116-126, 270-321)."""
     if gf[-2:].isdigit() and ("DD" in gf):
@@ -58,8 +96,10 @@ def parity_type(gf: str) -> int:
 class BSSNQuantities:
     """Conformal BSSN tensors built from the rescaled gridfunctions."""
 
-    def __init__(self, rfm: RefMetric, ricci_symbolic: bool):
+    def __init__(self, rfm: RefMetric, ricci_symbolic: bool, stager=None):
         self.rfm = rfm
+        st = stager or _nostage
+        self.st = st
         s = sp.Symbol
         self.hDD = T.declare("hDD", 2, "sym01")
         self.aDD = T.declare("aDD", 2, "sym01")
@@ -70,14 +110,14 @@ class BSSNQuantities:
         I3 = range(3)
 
         # gammabar_ij = h_ij ReDD_ij + ghat_ij ; Abar_ij = a_ij ReDD_ij
-        self.gammabarDD = [[self.hDD[i][j] * rfm.ReDD[i][j] + rfm.ghatDD[i][j] for j in I3] for i in I3]
-        self.AbarDD = [[self.aDD[i][j] * rfm.ReDD[i][j] for j in I3] for i in I3]
+        self.gammabarDD = st("gbDD", [[self.hDD[i][j] * rfm.ReDD[i][j] + rfm.ghatDD[i][j] for j in I3] for i in I3])
+        self.AbarDD = st("AbDD", [[self.aDD[i][j] * rfm.ReDD[i][j] for j in I3] for i in I3])
         self.LambdabarU = [self.lambdaU[i] * rfm.ReU[i] for i in I3]
         self.betaU = [self.vetU[i] * rfm.ReU[i] for i in I3]
         self.BU = [self.betU[i] * rfm.ReU[i] for i in I3]
 
-        self.gammabarUU, _ = T.symm_inverse3x3(self.gammabarDD)
-        gUU = self.gammabarUU
+        gUU, _ = T.symm_inverse3x3(self.gammabarDD)
+        self.gammabarUU = gUU = st("gbUU", gUU)
 
         hDD_dD = T.declare("hDD_dD", 3, "sym01")
         hDD_dupD = T.declare("hDD_dupD", 3, "sym01")
@@ -88,13 +128,14 @@ class BSSNQuantities:
             base = rfm.ghatDDdD[i][j][k] + self.hDD[i][j] * rfm.ReDDdD[i][j][k]
             self.gammabarDD_dD[i][j][k] = base + hDD_dD[i][j][k] * rfm.ReDD[i][j]
             self.gammabarDD_dupD[i][j][k] = base + hDD_dupD[i][j][k] * rfm.ReDD[i][j]
+        self.gammabarDD_dD = st("gbDDdD", self.gammabarDD_dD)
 
         # Gammabar^i_kl
         G = T.zeros(3)
         dgb = self.gammabarDD_dD
         for i, k, l in T.idx(3):
             G[i][k][l] = sum(R(1, 2) * gUU[i][m] * (dgb[m][k][l] + dgb[m][l][k] - dgb[k][l][m]) for m in I3)
-        self.GammabarUDD = G
+        self.GammabarUDD = G = st("GbUDD", G)
 
         # det(gammabar) = det(gammahat) (detgbarOverdetghat_equals_one).
         self.detgammabar = rfm.detgammahat
@@ -103,9 +144,10 @@ class BSSNQuantities:
 
         # Abar^{ij}, Abar^i_j, trace
         A = self.AbarDD
-        self.AbarUU = [[sum(gUU[i][k] * gUU[j][l] * A[k][l] for k in I3 for l in I3) for j in I3] for i in I3]
-        self.AbarUD = [[sum(gUU[i][k] * A[k][j] for k in I3) for j in I3] for i in I3]
-        self.trAbar = sum(gUU[k][j] * A[j][k] for k in I3 for j in I3)
+        self.AbarUD = st("AbUD", [[sum(gUU[i][k] * A[k][j] for k in I3) for j in I3] for i in I3])
+        self.AbarUU = st("AbUU", [[sum(self.AbarUD[i][l] * gUU[j][l] for l in I3) for j in I3] for i in I3]) \
+            if stager else [[sum(gUU[i][k] * gUU[j][l] * A[k][l] for k in I3 for l in I3) for j in I3] for i in I3]
+        self.trAbar = st("trAb", sum(gUU[k][j] * A[j][k] for k in I3 for j in I3))
         aDD_dD = T.declare("aDD_dD", 3, "sym01")
         aDD_dupD = T.declare("aDD_dupD", 3, "sym01")
         self.AbarDD_dD = T.zeros(3)
@@ -115,8 +157,8 @@ class BSSNQuantities:
             self.AbarDD_dD[i][j][k] = rfm.ReDDdD[i][j][k] * self.aDD[i][j] + rfm.ReDD[i][j] * aDD_dD[i][j][k]
 
         # Delta^i_jk = Gammabar - Gammahat ; Delta^i
-        self.DGammaUDD = [[[G[i][j][k] - rfm.GammahatUDD[i][j][k] for k in I3] for j in I3] for i in I3]
-        self.DGammaU = [sum(gUU[j][k] * self.DGammaUDD[i][j][k] for j in I3 for k in I3) for i in I3]
+        self.DGammaUDD = st("DGUDD", [[[G[i][j][k] - rfm.GammahatUDD[i][j][k] for k in I3] for j in I3] for i in I3])
+        self.DGammaU = st("DGU", [sum(gUU[j][k] * self.DGammaUDD[i][j][k] for j in I3 for k in I3) for i in I3])
 
         if ricci_symbolic:
             self.RbarDD = T.declare("RbarDD", 2, "sym01")
@@ -140,20 +182,22 @@ class BSSNQuantities:
                     + vetU_dD[i][k] * rfm.ReUdD[i][j]
                     + self.vetU[i] * rfm.ReUdDD[i][j][k]
                 )
+        self.betaU_dD = st("bUdD", self.betaU_dD)
+        self.betaU_dDD = st("bUdDD", self.betaU_dDD)
 
         # Conformal factor W = e^{-2 phi}
         cf = self.cf
         cf_dD = T.declare("cf_dD", 1)
         cf_dupD = T.declare("cf_dupD", 1)
         cf_dDD = T.declare("cf_dDD", 2, "sym01")
-        self.phi_dD = [-cf_dD[i] / (2 * cf) for i in I3]
+        self.phi_dD = st("phidD", [-cf_dD[i] / (2 * cf) for i in I3])
         self.phi_dupD = [-cf_dupD[i] / (2 * cf) for i in I3]
         self.phi_dDD = [[(-cf_dDD[i][j] + cf_dD[i] * cf_dD[j] / cf) / (2 * cf) for j in I3] for i in I3]
         self.exp_m4phi = cf * cf
         self.phi_dBarD = self.phi_dD
-        self.phi_dBarDD = [
+        self.phi_dBarDD = st("phidBDD", [
             [self.phi_dDD[i][j] - sum(G[k][i][j] * self.phi_dD[k] for k in I3) for j in I3] for i in I3
-        ]
+        ])
 
     def _ricci(self, hDD_dD, hDD_dDD):
         """Conformal Ricci tensor (BSSN_quantities.py:261-449)."""
@@ -161,7 +205,8 @@ class BSSNQuantities:
         I3 = range(3)
         gUU = self.gammabarUU
         Gh = rfm.GammahatUDD
-        eps = [[self.hDD[i][j] * rfm.ReDD[i][j] for j in I3] for i in I3]
+        st = self.st
+        eps = st("eps", [[self.hDD[i][j] * rfm.ReDD[i][j] for j in I3] for i in I3])
         eps_dD = T.zeros(3)
         for i, j, k in T.idx(3):
             eps_dD[i][j][k] = hDD_dD[i][j][k] * rfm.ReDD[i][j] + self.hDD[i][j] * rfm.ReDDdD[i][j][k]
@@ -173,13 +218,15 @@ class BSSNQuantities:
                 + hDD_dD[i][j][l] * rfm.ReDDdD[i][j][k]
                 + self.hDD[i][j] * rfm.ReDDdDD[i][j][k][l]
             )
+        eps_dD = st("epsdD", eps_dD)
+        eps_dDD = st("epsdDD", eps_dDD)
         # \bar{gamma}_{ij;\hat{l}}
         gDhat = T.zeros(3)
         for i, j, l in T.idx(3):
             gDhat[i][j][l] = eps_dD[i][j][l] + sum(
                 -Gh[m][i][l] * eps[m][j] - Gh[m][j][l] * eps[i][m] for m in I3
             )
-        self.gammabarDD_dHatD = gDhat
+        self.gammabarDD_dHatD = gDhat = st("gDh", gDhat)
         # \bar{gamma}_{ij;\hat{l},k}
         gDhat_dD = T.zeros(4)
         for i, j, l, k in T.idx(4):
@@ -190,6 +237,7 @@ class BSSNQuantities:
                 - Gh[m][j][l] * eps_dD[i][m][k]
                 for m in I3
             )
+        gDhat_dD = st("gDhdD", gDhat_dD)
         # \bar{gamma}_{ij;\hat{l}\hat{k}}
         gDhatDD = T.zeros(4)
         for i, j, l, k in T.idx(4):
@@ -197,6 +245,7 @@ class BSSNQuantities:
                 -Gh[m][l][k] * gDhat[i][j][m] - Gh[m][i][k] * gDhat[m][j][l] - Gh[m][j][k] * gDhat[i][m][l]
                 for m in I3
             )
+        gDhatDD = st("gDhDD", gDhatDD)
         # \hat{D}_j \bar{Lambda}^k
         lambdaU_dD = T.declare("lambdaU_dD", 2)
         LamDhat = T.zeros(2)
@@ -208,7 +257,8 @@ class BSSNQuantities:
             )
         DG = self.DGammaUDD
         gDD = self.gammabarDD
-        DGammaDDD = [[[sum(gDD[i][m] * DG[m][j][k] for m in I3) for k in I3] for j in I3] for i in I3]
+        LamDhat = st("LDh", LamDhat)
+        DGammaDDD = st("DGDDD", [[[sum(gDD[i][m] * DG[m][j][k] for m in I3) for k in I3] for j in I3] for i in I3])
         Rb = T.zeros(2)
         for i, j in T.idx(2):
             Rb[i][j] += sum(-R(1, 2) * gUU[k][l] * gDhatDD[i][j][l][k] for k in I3 for l in I3)
@@ -230,9 +280,9 @@ class BSSNQuantities:
         return Rb
 
 
-def ricci_exprs(coord: str) -> Dict[str, sp.Expr]:
+def ricci_exprs(coord: str, stager=None) -> Dict[str, sp.Expr]:
     """RbarDD{ij} (i<=j) in terms of hDD, its derivatives and lambdaU_dD."""
-    Bq = BSSNQuantities(reference_metric(coord), ricci_symbolic=False)
+    Bq = BSSNQuantities(reference_metric(coord), ricci_symbolic=False, stager=stager)
     return {f"RbarDD{i}{j}": Bq.RbarDD[i][j] for i in range(3) for j in range(i, 3)}
 
 
@@ -240,6 +290,7 @@ def bssn_rhs_exprs(
     coord: str,
     ricci_symbolic: bool = True,
     enable_KO: bool = True,
+    stager=None,
 ) -> Dict[str, sp.Expr]:
     """All 24 BSSN right-hand sides, keyed ``<gf>_rhs`` style names.
 
@@ -248,7 +299,8 @@ def bssn_rhs_exprs(
     ``bet_rhsU{i}``), and KO dissipation from ``rhs_eval.py:166-220``.
     """
     rfm = reference_metric(coord)
-    Bq = BSSNQuantities(rfm, ricci_symbolic=ricci_symbolic)
+    Bq = BSSNQuantities(rfm, ricci_symbolic=ricci_symbolic, stager=stager)
+    st = Bq.st
     I3 = range(3)
     gDD, gUU, A = Bq.gammabarDD, Bq.gammabarUU, Bq.AbarDD
     alpha, trK, cf = Bq.alpha, Bq.trK, Bq.cf
@@ -261,7 +313,7 @@ def bssn_rhs_exprs(
         gam_rhs[i][j] = sum(
             betaU[k] * Bq.gammabarDD_dupD[i][j][k] + bdD[k][i] * gDD[k][j] + bdD[k][j] * gDD[i][k] for k in I3
         )
-    Dbarbeta = sum(bdD[k][k] + betaU[k] * detg_dD[k] / (2 * detg) for k in I3)
+    Dbarbeta = st("Dbb", sum(bdD[k][k] + betaU[k] * detg_dD[k] / (2 * detg) for k in I3))
     for i, j in T.idx(2):
         gam_rhs[i][j] += R(2, 3) * gDD[i][j] * (alpha * Bq.trAbar - Dbarbeta)
     for i, j in T.idx(2):
@@ -279,7 +331,7 @@ def bssn_rhs_exprs(
     alpha_dD = T.declare("alpha_dD", 1)
     alpha_dDD = T.declare("alpha_dDD", 2, "sym01")
     G = Bq.GammabarUDD
-    alpha_dBarDD = [[alpha_dDD[i][j] - sum(G[k][i][j] * alpha_dD[k] for k in I3) for j in I3] for i in I3]
+    alpha_dBarDD = st("adBDD", [[alpha_dDD[i][j] - sum(G[k][i][j] * alpha_dD[k] for k in I3) for j in I3] for i in I3])
     phi_dD, phi_dBarDD = Bq.phi_dBarD, Bq.phi_dBarDD
     curly = T.zeros(2)
     for i, j in T.idx(2):
@@ -291,7 +343,8 @@ def bssn_rhs_exprs(
             - alpha_dBarDD[i][j]
             + alpha * Bq.RbarDD[i][j]
         )
-    curly_tr = sum(gUU[i][j] * curly[i][j] for i in I3 for j in I3)
+    curly = st("curly", curly)
+    curly_tr = st("curlytr", sum(gUU[i][j] * curly[i][j] for i in I3 for j in I3))
     for i, j in T.idx(2):
         A_rhs[i][j] += Bq.exp_m4phi * (curly[i][j] - R(1, 3) * gDD[i][j] * curly_tr)
 
