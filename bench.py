@@ -9,6 +9,8 @@ value = interior points x timed RK4 steps / seconds of MoL_step_forward_in_time,
 with all fields resident in HBM.  Prints one JSON line.
 
     python bench.py [--gpus 1] [--steps 100] [--warmup 5] [--config c3|c2] [--no-cpu]
+    python bench.py --copy-calibration 512   (plain 512 MiB device copy x3, for
+                                              scaling the FETCH_SIZE/WRITE_SIZE counters)
 """
 
 from __future__ import annotations
@@ -31,6 +33,17 @@ CONFIGS = {
 }
 PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector (= FP64 matrix) peak, dense
 PEAK_HBM_GBS = 8000.0
+
+# Memory-side bytes per rhs_eval launch (all kernels of the schedule), from
+# rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of this bench, scaled by
+# the same counters on a plain 512 MiB device copy (FETCH_SIZE x2.000, WRITE_SIZE
+# x1.000); summary and commands in profiles/bssn_sinhspherical_192x96x96_hbm_traffic.md.
+MEASURED_TRAFFIC = {
+    ("SinhSpherical_symnone_KO1_fd6", (192, 96, 96), "p3_lb1_sl"): {
+        "fetch_bytes": 2 * (625803 + 833376 + 1026360) * 1024.0,
+        "write_bytes": (152064 + 96768 + 387072) * 1024.0,
+    },
+}
 
 
 def cpu_baseline(c: dict, steps: int = 1) -> dict:
@@ -70,6 +83,8 @@ def main() -> None:
     ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB three times, then exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -82,8 +97,17 @@ def main() -> None:
     from nrpy_amd import initial_data as ID
     from nrpy_amd.runtime import MoLState, Patch, PatchConfig
 
-    c = CONFIGS[args.config]
     torch.cuda.set_device(0)
+    if args.copy_calibration:
+        n = args.copy_calibration * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_each_way": n * 8, "copies": 3}))
+        return
+    c = CONFIGS[args.config]
     cfg = PatchConfig(coord=c["coord"], Nxx=c["Nxx"], fd_order=c["fd_order"], symmetry_axes=c["sym"],
                       enable_KO=c["KO"], grid_physical_size=c["size"], SINHW=c["sinhw"], eta=2.0,
                       KO_strength_gauge=0.3, KO_strength_nongauge=0.3)
@@ -135,6 +159,17 @@ def main() -> None:
     torch.cuda.synchronize()
     ricci_ms = e0.elapsed_time(e1) / reps
     achieved = flops_rhs * npts / (rhs_ms * 1e-3) / 1e12
+    from nrpy_amd.runtime import KERNEL_RHS
+    from nrpy_amd.autotune import build_tag
+
+    sched = P.schedule(KERNEL_RHS)
+    measured = MEASURED_TRAFFIC.get((build_tag(cfg), tuple(c["Nxx"]), sched))
+    # algorithmic bytes: every input value read once (24 evolved + 6 Ricci gridfunctions,
+    # ghost zones included), every right-hand side written once
+    ntot = 1
+    for n in c["Nxx"]:
+        ntot *= n + 2 * (c["fd_order"] // 2 + 1)
+    alg_bytes = 8.0 * (30 * ntot + 24 * npts)
     result = {
         "metric": "BSSN gridpoint-RK4-steps/s FP64 on 1 GPU",
         "value": value,
@@ -159,12 +194,13 @@ def main() -> None:
         "roofline": {
             "bound": "mfma",
             "compute": "FP64 VALU (stencil arithmetic, no MFMA); peak is MI355X dense FP64",
-            "kernel": f"bssn_rhs_{tag}_fd{c['fd_order']}",
+            "kernel": f"rhs_eval: bssn_rhs_{tag}_fd{c['fd_order']}_{sched}_g*",
             "achieved": achieved,
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_TFLOPS,
-            "traffic": None,
+            "traffic": (measured["fetch_bytes"] + measured["write_bytes"]) if measured else None,
+            "algorithmic_bytes": alg_bytes,
             "flops_per_point": flops_rhs,
             "avg_launch_ms": rhs_ms,
             "ricci_avg_launch_ms": ricci_ms,
