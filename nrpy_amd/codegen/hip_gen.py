@@ -64,8 +64,28 @@ class StencilEmitter:
         self.fd_order = fd_order
         self.array = array
         self.R = fd.ghost_zones(fd_order)
+        self._collect: List[Tuple[str, str]] | None = None
+        self._nload = 0
+
+    def split(self, dv: fd.Deriv) -> Tuple[List[Tuple[str, str]], str]:
+        """(loads as (register name, load expression), combine expression over those names)."""
+        self._collect = []
+        try:
+            e = self.expr(dv)
+            return self._collect, e
+        finally:
+            self._collect = None
 
     def load(self, gf: str, off: Tuple[int, int, int]) -> str:
+        if self._collect is not None:
+            self._collect, coll = None, self._collect
+            try:
+                name = f"ld{self._nload}"
+                self._nload += 1
+                coll.append((name, self.load(gf, off)))
+                return name
+            finally:
+                self._collect = coll
         g = self.spec.gfs.index(gf)
         terms = []
         if off[0]:
@@ -228,7 +248,8 @@ def _emit_defs(spec: specs.Spec, fd_order: int):
 
 def _ordered_body(spec: specs.Spec, fd_order: int, out_lvalues: List[str], rationals: Dict[str, str],
                   stream_linear: bool = True, order: str = "dfs", barriers: bool = True,
-                  cse_scope: str = "joint") -> str:
+                  cse_scope: str = "joint", linear: str = "stream", prefetch: int = 0,
+                  ahead: int = 12) -> str:
     """Stencils and CSE'd algebra for one pointwise spec.
 
     With ``stream_linear``, upwind and KO derivatives are consumed as soon as
@@ -236,50 +257,144 @@ def _ordered_body(spec: specs.Spec, fd_order: int, out_lvalues: List[str], ratio
     remaining core algebra is CSE'd; centre values, centred derivatives and
     CSE temporaries are then emitted in first-use order.  Scheduling barriers
     after each stencil keep the compiler from hoisting every load to the top.
+
+    ``linear="lines"`` evaluates an output's upwind and KO terms just before
+    the output is stored, one 1D line of 2R points per (gridfunction,
+    direction) loaded once and shared by the two one-sided upwind sums and the
+    KO sum (the upwind stencils and the KO stencil cover the same line).
+
+    ``prefetch=B`` issues each stencil's loads up to ``ahead`` barrier regions
+    before the region that combines them, in consumption order, keeping at
+    most B early-issued values in flight, so that load latency overlaps the
+    arithmetic of earlier regions instead of stalling the single wave per SIMD.
     """
     defs = _emit_defs(spec, fd_order)
     derivs = {d.symbol: d for d in spec.classify()[2]}
+    centre_names = set(spec.classify()[0]) | set(spec.classify()[1])
     pr = emit.DoublePrinter(rationals)
+    st = StencilEmitter(spec, fd_order)
     emitted: Set[str] = set()
-    lines: List[str] = []
+    items: List[tuple] = []  # ("C", text) | ("L", text, group) | ("B",)
     flops = [0]
+    gid = [0, 0]  # load-group counter, line counter
+    split_loads = prefetch > 0
+
+    def code(t):
+        items.append(("C", t))
+
+    def barrier():
+        if barriers:
+            items.append(("B",))
+
+    def loads(pairs):
+        gid[0] += 1
+        for name, e in pairs:
+            items.append(("L", f"  const double {name} = {e};", gid[0]))
 
     def need(expr, indent="  "):
-        for s in sorted(expr.free_symbols, key=str):
-            n = str(s)
+        for sy in sorted(expr.free_symbols, key=str):
+            n = str(sy)
             if n in defs and n not in emitted:
                 emitted.add(n)
                 if n in derivs:
                     flops[0] += stencil_flops(derivs[n], fd_order)
-                lines.append(f"{indent}const double {n} = {defs[n]};")
-                if barriers and fd.is_deriv_symbol(n):
-                    lines.append(f"{indent}__builtin_amdgcn_sched_barrier(0);")
+                    if split_loads:
+                        ld, comb = st.split(derivs[n])
+                        loads(ld)
+                        code(f"{indent}const double {n} = {comb};")
+                    else:
+                        code(f"{indent}const double {n} = {defs[n]};")
+                    barrier()
+                elif split_loads and n in centre_names:
+                    loads([(n, defs[n])])
+                else:
+                    code(f"{indent}const double {n} = {defs[n]};")
 
     outs = [e for _, e in spec.outputs]
     acc_names: Dict[str, str] = {}
+    lin_terms: Dict[str, list] = {}
+    cname: Dict = {}
     if stream_linear:
         cores, terms = split_linear(spec)
         if terms:
             coefs = sorted({c for tl in terms.values() for _, c in tl}, key=sp.default_sort_key)
             crep, cred = sp.cse(coefs, symbols=sp.numbered_symbols("lc"), order="none")
-            for s, e in crep:
+            for sy, e in crep:
                 need(e)
-                lines.append(f"  const double {s} = {pr.doprint(e)};")
-            cname = {}
+                code(f"  const double {sy} = {pr.doprint(e)};")
             for k, (c, e) in enumerate(zip(coefs, cred)):
                 need(e)
-                lines.append(f"  const double lcoef{k} = {pr.doprint(e)};")
+                code(f"  const double lcoef{k} = {pr.doprint(e)};")
                 cname[c] = f"lcoef{k}"
             flops[0] += sum(emit._flops(e) for _, e in crep) + sum(emit._flops(e) for e in cred)
-            for g, tl in terms.items():
-                acc = f"lin_{g}"
-                acc_names[g] = acc
-                lines.append(f"  double {acc} = 0.0;")
-                for d, c in sorted(tl, key=lambda t: t[0].symbol):
-                    lines.append(f"  {acc} += {cname[c]} * ({defs[d.symbol]});")
-                    flops[0] += 2 + stencil_flops(d, fd_order)
-                lines.append("  __builtin_amdgcn_sched_barrier(0);")
+            if linear == "stream":
+                for g, tl in terms.items():
+                    acc = f"lin_{g}"
+                    acc_names[g] = acc
+                    code(f"  double {acc} = 0.0;")
+                    for d, c in sorted(tl, key=lambda t: t[0].symbol):
+                        code(f"  {acc} += {cname[c]} * ({defs[d.symbol]});")
+                        flops[0] += 2 + stencil_flops(d, fd_order)
+                    if barriers:
+                        items.append(("B",))
+            else:
+                lin_terms = terms
         outs = cores
+
+    R = fd.ghost_zones(fd_order)
+    w_up = dict(fd.weights_1d("dupD", fd_order))
+    w_dn = dict(fd.weights_1d("ddnD", fd_order))
+    w_ko = dict(fd.weights_1d("dKOD", fd_order))
+
+    def emit_lines(g):
+        """lin_<g> from shared line loads (linear == "lines")."""
+        acc = f"lin_{g}"
+        acc_names[g] = acc
+        code(f"  double {acc} = 0.0;")
+        per: Dict[Tuple[str, int], Dict[str, str]] = {}
+        for d, c in terms_sorted(lin_terms[g]):
+            per.setdefault((d.gf, d.dirs[0]), {})["up" if d.op == "dupD" else "ko"] = cname[c]
+        for (gf, dirn), cf in sorted(per.items()):
+            gid[1] += 1
+            if gf in defs:
+                need(sp.Symbol(gf, real=True))
+                f0 = gf
+            else:
+                f0 = f"ctr{gid[1]}_{gf}"
+                if split_loads:
+                    loads([(f0, st.load(gf, (0, 0, 0)))])
+                else:
+                    code(f"  const double {f0} = {st.load(gf, (0, 0, 0))};")
+            pts = {}
+            ld = []
+            for k in range(-R, R + 1):
+                if k == 0:
+                    continue
+                o = [0, 0, 0]
+                o[dirn] = k
+                name = f"ln{gid[1]}_{k + R}"
+                ld.append((name, st.load(gf, tuple(o))))
+                pts[k] = name
+            pts[0] = f0
+            if split_loads:
+                loads(ld)
+            else:
+                for name, e in ld:
+                    code(f"  const double {name} = {e};")
+            terms_c = []
+            if "up" in cf:
+                su = " + ".join(f"{_dbl(w)} * {pts[k]}" for k, w in sorted(w_up.items()))
+                sd = " + ".join(f"{_dbl(w)} * {pts[k]}" for k, w in sorted(w_dn.items()))
+                terms_c.append(f"{cf['up']} * (p.invdxx{dirn} * ((betaU{dirn} > 0.0) ? ({su}) : ({sd})))")
+                flops[0] += 2 * (2 * len(w_up) - 1) + 2
+            if "ko" in cf:
+                sk = [f"{_dbl(w_ko[0])} * {pts[0]}"]
+                sk += [f"{_dbl(w_ko[k])} * ({pts[k]} + {pts[-k]})" for k in sorted(w_ko) if k > 0]
+                terms_c.append(f"{cf['ko']} * (p.invdxx{dirn} * ({' + '.join(sk)}))")
+                flops[0] += 3 * (len(w_ko) // 2) + 3
+            code(f"  {acc} += {' + '.join(terms_c)};")
+            barrier()
+
     sdefs = spec.used_defs()
     if sdefs and cse_scope == "local":
         # CSE within each staged definition and each output separately: temporaries
@@ -306,14 +421,14 @@ def _ordered_body(spec: specs.Spec, fd_order: int, out_lvalues: List[str], ratio
             replaced = list(replaced) + staged_defs
             order = "dfs"
     if order == "cse":
-        for s, e in replaced:
+        for sy, e in replaced:
             need(e)
-            lines.append(f"  const double {s} = {pr.doprint(e)};")
+            code(f"  const double {sy} = {pr.doprint(e)};")
             flops[0] += emit._flops(e)
     else:
         # Demand-driven depth-first emission: each output pulls in the temporaries it
         # needs just before it is stored, so values used by one output die early.
-        tdef = {s: e for s, e in replaced}
+        tdef = {sy: e for sy, e in replaced}
         done: Set = set()
 
         def emit_t(sym):
@@ -325,7 +440,7 @@ def _ordered_body(spec: specs.Spec, fd_order: int, out_lvalues: List[str], ratio
                 if expanded:
                     done.add(t)
                     need(tdef[t])
-                    lines.append(f"  const double {t} = {pr.doprint(tdef[t])};")
+                    code(f"  const double {t} = {pr.doprint(tdef[t])};")
                     flops[0] += emit._flops(tdef[t])
                     continue
                 stack.append((t, True))
@@ -333,15 +448,87 @@ def _ordered_body(spec: specs.Spec, fd_order: int, out_lvalues: List[str], ratio
                     stack.append((d, False))
 
     for (g, _), lv, e in zip(spec.outputs, out_lvalues, reduced):
+        if g in lin_terms:
+            emit_lines(g)
         if order != "cse":
             for d in sorted((x for x in e.free_symbols if x in tdef), key=str):
                 emit_t(d)
         need(e)
         tail = f" + {acc_names[g]}" if g in acc_names else ""
-        lines.append(f"  {lv} = {pr.doprint(e)}{tail};")
+        code(f"  {lv} = {pr.doprint(e)}{tail};")
         flops[0] += emit._flops(e) + (1 if tail else 0)
-    FLOPS[f"{spec.name}_fd{fd_order}"] = flops[0]
-    return "\n".join(lines)
+    FLOPS[f"{spec.name}_fd{fd_order}{_mode_suffix(linear, prefetch)}"] = flops[0]
+    if prefetch:
+        items = _schedule_prefetch(items, prefetch, ahead)
+    out = []
+    for it in items:
+        out.append("  __builtin_amdgcn_sched_barrier(0);" if it[0] == "B" else it[1])
+    return "\n".join(out)
+
+
+def terms_sorted(tl):
+    return sorted(tl, key=lambda t: t[0].symbol)
+
+
+def _mode_suffix(linear: str, prefetch: int) -> str:
+    return ("" if linear == "stream" else f"_{linear}") + (f"_pf{prefetch}" if prefetch else "")
+
+
+def _schedule_prefetch(items: List[tuple], budget: int, ahead: int) -> List[tuple]:
+    """Move each group of loads to the earliest region allowed by the budget.
+
+    Regions are the stretches between scheduling barriers.  Groups keep their
+    consumption order (so a wait for one group never waits for a later one);
+    a group consumed in region c may start in region s >= c - ahead when every
+    region in [s, c) holds fewer than ``budget`` early values including it.
+    """
+    regions: List[List[tuple]] = [[]]
+    for it in items:
+        if it[0] == "B":
+            regions.append([])
+        else:
+            regions[-1].append(it)
+    nreg = len(regions)
+    inflight = [0] * nreg
+    early: List[List[tuple]] = [[] for _ in range(nreg)]
+    keep: List[List[tuple]] = [[] for _ in range(nreg)]
+    own: List[List[tuple]] = [[] for _ in range(nreg)]
+    prev_s = 0
+    for c, reg in enumerate(regions):
+        groups: Dict[int, List[tuple]] = {}
+        order_g: List[int] = []
+        for it in reg:
+            if it[0] == "L":
+                if it[2] not in groups:
+                    groups[it[2]] = []
+                    order_g.append(it[2])
+                groups[it[2]].append(it)
+            else:
+                keep[c].append(it)
+        # a region's own loads are issued first, then the early loads of later
+        # regions, then its code: issue order equals consumption order
+        placed_here: List[tuple] = []
+        for g in order_g:
+            n = len(groups[g])
+            sreg = c
+            while sreg - 1 >= max(prev_s, c - ahead) and inflight[sreg - 1] + n <= budget:
+                sreg -= 1
+            for r in range(sreg, c):
+                inflight[r] += n
+            prev_s = sreg
+            if sreg == c:
+                placed_here.extend(groups[g])
+            else:
+                early[sreg].extend(groups[g])
+        own[c] = placed_here
+    out: List[tuple] = []
+    for r in range(nreg):
+        if r:
+            out.append(("B",))
+        out.extend(own[r])
+        out.extend(early[r])
+        out.extend(keep[r])
+    return out
 
 
 KPARAMS = r"""
@@ -356,8 +543,33 @@ struct bhah_kparams {
 """
 
 
-def _prologue(interior: bool) -> str:
+XCD_REMAP = """  // Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2):
+  // renumber them so XCD k sweeps the k-th contiguous eighth of the tiles and
+  // stencil neighbours share an L2.  Identity when the tile count is not a multiple of 8.
+  unsigned bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+  {
+    const unsigned gx = gridDim.x, gy = gridDim.y, n = gx * gy * gridDim.z;
+    if ((n & 7u) == 0u) {
+      const unsigned h = bx + gx * (by + gy * bz);
+      const unsigned l = (h & 7u) * (n >> 3) + (h >> 3);
+      bx = l % gx;
+      by = (l / gx) % gy;
+      bz = l / (gx * gy);
+    }
+  }
+"""
+
+
+def _prologue(interior: bool, xcd: bool = False) -> str:
     lo = "p.NG" if interior else "0"
+    if xcd:
+        return XCD_REMAP + f"""  const int i0 = {lo} + (int)(bx * blockDim.x + threadIdx.x);
+  const int i1 = {lo} + (int)(by * blockDim.y + threadIdx.y);
+  const int i2 = {lo} + (int)bz;
+  if (i0 >= p.Nx0 - {lo} || i1 >= p.Nx1 - {lo}) return;
+  const long s1 = p.s1, s2 = p.s2, Ntot = p.Ntot;
+  const long idx = i0 + s1 * i1 + s2 * i2;
+"""
     return f"""  const int i0 = {lo} + (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const int i1 = {lo} + (int)(blockIdx.y * blockDim.y + threadIdx.y);
   const int i2 = {lo} + (int)blockIdx.z;
@@ -367,7 +579,7 @@ def _prologue(interior: bool) -> str:
 """
 
 
-def _upwind_prologue(spec: specs.Spec, fd_order: int, rationals) -> str:
+def _upwind_prologue(spec: specs.Spec, fd_order: int, rationals, tables: bool = True) -> str:
     st = StencilEmitter(spec, fd_order)
     pr = emit.DoublePrinter(rationals)
     lines = []
@@ -380,7 +592,8 @@ def _upwind_prologue(spec: specs.Spec, fd_order: int, rationals) -> str:
     sub = {sp.Symbol(n, real=True): sp.Symbol("up_" + n, real=True) for n in used}
     for d in range(3):
         lines.append(f"  const double betaU{d} = {pr.doprint(sp.sympify(spec.upwind[d]).xreplace(sub))};")
-    lines.append(st.upwind_tables())
+    if tables:
+        lines.append(st.upwind_tables())
     return "\n".join(lines)
 
 
@@ -462,12 +675,14 @@ LAUNCH_BOUNDS = {"lb1": "256", "lb2": "256, 2"}
 
 # Candidate schedules per build: (kind, partition, launch bounds).
 FULL_CANDIDATES = {
-    "rhs": ["p1_lb1", "p1_lb1_sj", "p1_lb1_sl", "p1_lb2_sl", "p2_lb1_sl", "p3_lb1", "p3_lb1_sl", "p3_lb2_sl",
-            "p4_lb1"],
-    "ricci": ["r1_lb1", "r1_lb1_sj", "r1_lb1_sl", "r1_lb2_sl", "r2_lb1", "r2_lb1_sl", "r2_lb2_sl", "r3_lb1",
-              "r3_lb2_sl", "r6_lb1"],
+    "rhs": ["p1_lb1", "p1_lb1_sj", "p2_lb1_sl", "p3_lb1", "p3_lb1_sl", "p4_lb1", "p3_lb1_sl_xcd",
+            "p3_lb1_sl_ln", "p3_lb1_sl_ln_xcd", "p3_lb1_sl_ln_pf16", "p3_lb1_sl_ln_pf32", "p3_lb1_sl_ln_pf16_xcd",
+            "p2_lb1_sl_ln_pf16", "p4_lb1_sl_ln_pf16"],
+    "ricci": ["r1_lb1", "r1_lb1_sj", "r2_lb1", "r2_lb1_sl", "r3_lb1", "r3_lb1_xcd", "r3_lb1_pf16", "r3_lb1_pf32",
+              "r3_lb1_pf16_xcd", "r2_lb1_sl_pf16", "r2_lb1_sl_pf32", "r6_lb1_pf32"],
 }
-SMALL_CANDIDATES = {"rhs": ["p1_lb1", "p1_lb1_sl", "p3_lb1_sl"], "ricci": ["r1_lb1", "r1_lb1_sl", "r2_lb1_sl"]}
+SMALL_CANDIDATES = {"rhs": ["p1_lb1", "p1_lb1_sl", "p3_lb1_sl", "p1_lb1_sl_ln_pf16", "p3_lb1_sl_ln_pf16"],
+                    "ricci": ["r1_lb1", "r1_lb1_sl", "r2_lb1_sl", "r1_lb1_pf16", "r2_lb1_sl_pf16"]}
 
 CAND_SIG = ("dim3 g, dim3 b, hipStream_t s, const double *in, const double *aux, double *out, "
             "const double *rfm0, const double *rfm1, bhah_kparams p")
@@ -488,9 +703,13 @@ def candidate_tu(v: specs.Variant, fd_order: int, kind: str, cand: str) -> Tuple
     tag = f"{v.tag}_fd{fd_order}"
     parts = cand.split("_")
     part, lb = parts[0], parts[1]
-    mode = parts[2] if len(parts) > 2 else "x"
+    opts = parts[2:]
+    mode = next((o for o in opts if o in ("x", "sj", "sl")), "x")
     staged = mode in ("sj", "sl")
     scope = "local" if mode == "sl" else "joint"
+    linear = "lines" if "ln" in opts else "stream"
+    prefetch = next((int(o[2:]) for o in opts if o.startswith("pf")), 0)
+    xcd = "xcd" in opts
     rats: Dict[str, str] = {}
     kernels, calls, total = [], [], 0
     if kind == "rhs":
@@ -506,19 +725,19 @@ def candidate_tu(v: specs.Variant, fd_order: int, kind: str, cand: str) -> Tuple
         kname = f"bssn_{kind}_{tag}_{cand}_g{k}"
         if kind == "rhs":
             lv = [f"out[{bssn.EVOL_GFS.index(g)} * Ntot + idx]" for g, _ in sub.outputs]
-            body = _ordered_body(sub, fd_order, lv, rats, cse_scope=scope)
+            body = _ordered_body(sub, fd_order, lv, rats, cse_scope=scope, linear=linear, prefetch=prefetch)
             has_up = any(d.op == "dupD" for d in sub.classify()[2])
-            upw = _upwind_prologue(sub, fd_order, rats) if has_up else ""
+            upw = _upwind_prologue(sub, fd_order, rats, tables=(linear == "stream")) if has_up else ""
         else:
             lv = [f"out[{bssn.AUXEVOL_GFS.index(g)} * Ntot + idx]" for g, _ in sub.outputs]
-            body = _ordered_body(sub, fd_order, lv, rats, cse_scope=scope)
+            body = _ordered_body(sub, fd_order, lv, rats, cse_scope=scope, prefetch=prefetch)
             upw = ""
-        total += FLOPS[f"{sub.name}_fd{fd_order}"]
+        total += FLOPS[f"{sub.name}_fd{fd_order}{_mode_suffix(linear if kind == 'rhs' else 'stream', prefetch)}"]
         kernels.append(f"""
 extern "C" __global__ void __launch_bounds__({LAUNCH_BOUNDS[lb]})
 {kname}(const double *__restrict__ in, const double *__restrict__ aux, double *__restrict__ out,
         const double *__restrict__ rfm0, const double *__restrict__ rfm1, const bhah_kparams p) {{
-{_prologue(True)}
+{_prologue(True, xcd)}
 {upw}
 {body}
 }}
