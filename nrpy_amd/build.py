@@ -3,7 +3,9 @@
 Each generated kernel translation unit (``csrc/gen/*.hip``) and the host
 runtime (``csrc/bhah_hip.hip``) are compiled by ``hipcc`` in parallel, then
 linked.  ``python -m nrpy_amd.build [--regen]``; ``--regen`` first re-emits the
-kernels from the SymPy transcription.
+kernels from the SymPy transcription.  Schedules with early-issued loads are
+also compiled to assembly and refused if they can keep more vector-memory
+operations in flight than the 6-bit vmcnt counter tracks (``isa_check``).
 """
 
 from __future__ import annotations
@@ -14,6 +16,8 @@ import shutil
 import subprocess
 import sys
 from typing import List
+
+from . import isa_check
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -46,6 +50,15 @@ def _compile(src: str) -> str:
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr[-4000:]}")
+    if "_pf" in os.path.basename(src):
+        # early-issued stencil loads: verify the vmcnt range on the emitted assembly
+        asm = obj[:-2] + ".s"
+        r = subprocess.run([HIPCC, *HIPFLAGS, "--cuda-device-only", "-S", src, "-o", asm], capture_output=True,
+                           text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"hipcc -S failed for {src}:\n{r.stderr[-4000:]}")
+        with open(asm) as f:
+            isa_check.check(f.read(), src)
     return obj
 
 

@@ -383,9 +383,12 @@ def _ordered_body(spec: specs.Spec, fd_order: int, out_lvalues: List[str], ratio
                     code(f"  const double {name} = {e};")
             terms_c = []
             if "up" in cf:
+                # both one-sided sums, then a select (no divergent branch)
                 su = " + ".join(f"{_dbl(w)} * {pts[k]}" for k, w in sorted(w_up.items()))
                 sd = " + ".join(f"{_dbl(w)} * {pts[k]}" for k, w in sorted(w_dn.items()))
-                terms_c.append(f"{cf['up']} * (p.invdxx{dirn} * ((betaU{dirn} > 0.0) ? ({su}) : ({sd})))")
+                code(f"  const double su{gid[1]} = {su};")
+                code(f"  const double sd{gid[1]} = {sd};")
+                terms_c.append(f"{cf['up']} * (p.invdxx{dirn} * ((betaU{dirn} > 0.0) ? su{gid[1]} : sd{gid[1]}))")
                 flops[0] += 2 * (2 * len(w_up) - 1) + 2
             if "ko" in cf:
                 sk = [f"{_dbl(w_ko[0])} * {pts[0]}"]
@@ -675,11 +678,11 @@ LAUNCH_BOUNDS = {"lb1": "256", "lb2": "256, 2"}
 
 # Candidate schedules per build: (kind, partition, launch bounds).
 FULL_CANDIDATES = {
-    "rhs": ["p1_lb1", "p1_lb1_sj", "p2_lb1_sl", "p3_lb1", "p3_lb1_sl", "p4_lb1", "p3_lb1_sl_xcd",
-            "p3_lb1_sl_ln", "p3_lb1_sl_ln_xcd", "p3_lb1_sl_ln_pf16", "p3_lb1_sl_ln_pf32", "p3_lb1_sl_ln_pf16_xcd",
-            "p2_lb1_sl_ln_pf16", "p4_lb1_sl_ln_pf16"],
-    "ricci": ["r1_lb1", "r1_lb1_sj", "r2_lb1", "r2_lb1_sl", "r3_lb1", "r3_lb1_xcd", "r3_lb1_pf16", "r3_lb1_pf32",
-              "r3_lb1_pf16_xcd", "r2_lb1_sl_pf16", "r2_lb1_sl_pf32", "r6_lb1_pf32"],
+    "rhs": ["p1_lb1", "p3_lb1_sl", "p3_lb1_sl_ln", "p3_lb1_sl_ln_pf16", "p3_lb1_sl_ln_pf8", "p3_lb1_sl_ln_pf12",
+            "p3_lb1_sl_ln_pf24", "p3_lb1_sl_ln_pf16_a6", "p3_lb1_sl_ln_pf16_a24", "p3_lb1_ln_pf16",
+            "p3_lb1_sj_ln_pf16", "p2_lb1_sl_ln_pf8", "p4_lb1_sl_ln_pf12"],
+    "ricci": ["r1_lb1", "r3_lb1", "r2_lb1_sl_pf32", "r2_lb1_sl_pf24", "r2_lb1_sl_pf32_a6", "r2_lb1_sl_pf32_a24",
+              "r3_lb1_sl_pf32", "r2_lb1_sj_pf32", "r3_lb1_pf16"],
 }
 SMALL_CANDIDATES = {"rhs": ["p1_lb1", "p1_lb1_sl", "p3_lb1_sl", "p1_lb1_sl_ln_pf16", "p3_lb1_sl_ln_pf16"],
                     "ricci": ["r1_lb1", "r1_lb1_sl", "r2_lb1_sl", "r1_lb1_pf16", "r2_lb1_sl_pf16"]}
@@ -709,6 +712,7 @@ def candidate_tu(v: specs.Variant, fd_order: int, kind: str, cand: str) -> Tuple
     scope = "local" if mode == "sl" else "joint"
     linear = "lines" if "ln" in opts else "stream"
     prefetch = next((int(o[2:]) for o in opts if o.startswith("pf")), 0)
+    ahead = next((int(o[1:]) for o in opts if o[0] == "a" and o[1:].isdigit()), 12)
     xcd = "xcd" in opts
     rats: Dict[str, str] = {}
     kernels, calls, total = [], [], 0
@@ -725,12 +729,13 @@ def candidate_tu(v: specs.Variant, fd_order: int, kind: str, cand: str) -> Tuple
         kname = f"bssn_{kind}_{tag}_{cand}_g{k}"
         if kind == "rhs":
             lv = [f"out[{bssn.EVOL_GFS.index(g)} * Ntot + idx]" for g, _ in sub.outputs]
-            body = _ordered_body(sub, fd_order, lv, rats, cse_scope=scope, linear=linear, prefetch=prefetch)
+            body = _ordered_body(sub, fd_order, lv, rats, cse_scope=scope, linear=linear, prefetch=prefetch,
+                                 ahead=ahead)
             has_up = any(d.op == "dupD" for d in sub.classify()[2])
             upw = _upwind_prologue(sub, fd_order, rats, tables=(linear == "stream")) if has_up else ""
         else:
             lv = [f"out[{bssn.AUXEVOL_GFS.index(g)} * Ntot + idx]" for g, _ in sub.outputs]
-            body = _ordered_body(sub, fd_order, lv, rats, cse_scope=scope, prefetch=prefetch)
+            body = _ordered_body(sub, fd_order, lv, rats, cse_scope=scope, prefetch=prefetch, ahead=ahead)
             upw = ""
         total += FLOPS[f"{sub.name}_fd{fd_order}{_mode_suffix(linear if kind == 'rhs' else 'stream', prefetch)}"]
         kernels.append(f"""

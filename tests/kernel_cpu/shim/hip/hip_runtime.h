@@ -1,0 +1,38 @@
+/* TEST INFRASTRUCTURE ONLY.  Lets a generated kernel translation unit compile
+ * as plain host C++ so its kernels can run on the CPU (one call per thread
+ * index), e.g. under AddressSanitizer to check every memory access of every
+ * generated schedule against exact-size buffers. */
+#pragma once
+#include <stddef.h>
+
+struct dim3 {
+  unsigned x, y, z;
+  dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
+};
+typedef int hipError_t;
+typedef void *hipStream_t;
+static const hipError_t hipSuccess = 0;
+inline dim3 blockIdx, threadIdx, blockDim, gridDim;
+inline hipError_t hipGetLastError() { return hipSuccess; }
+
+#define __global__
+#define __device__
+#define __launch_bounds__(...)
+#define __builtin_amdgcn_sched_barrier(x) ((void)0)
+
+template <typename K, typename... A>
+void cpu_launch(K kernel, dim3 g, dim3 b, A... args) {
+  gridDim = g;
+  blockDim = b;
+  for (unsigned bz = 0; bz < g.z; bz++)
+    for (unsigned by = 0; by < g.y; by++)
+      for (unsigned bx = 0; bx < g.x; bx++)
+        for (unsigned tz = 0; tz < b.z; tz++)
+          for (unsigned ty = 0; ty < b.y; ty++)
+            for (unsigned tx = 0; tx < b.x; tx++) {
+              blockIdx = dim3(bx, by, bz);
+              threadIdx = dim3(tx, ty, tz);
+              kernel(args...);
+            }
+}
+#define hipLaunchKernelGGL(k, g, b, shmem, stream, ...) cpu_launch(k, g, b, __VA_ARGS__)

@@ -1,0 +1,112 @@
+"""Generated rhs/Ricci schedules executed on the CPU.
+
+Each schedule's translation unit is compiled as host C++ against a shim HIP
+header (tests/kernel_cpu/) with AddressSanitizer, every array an exact-size
+heap block, and run over the whole launch grid of a small patch; the outputs
+must match the oracle to 1e-12 and any out-of-bounds access aborts the run.
+This checks the emitted kernel text itself (stencil offsets, prefetch
+placement, launch guards) without a GPU.
+"""
+
+import os
+import re
+import shutil
+import subprocess
+import tempfile
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+import pytest
+
+from nrpy_amd import initial_data as ID
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GEN = os.path.join(os.path.dirname(HERE), "nrpy_amd", "csrc", "gen")
+SHIM = os.path.join(HERE, "kernel_cpu", "shim")
+DRIVER = os.path.join(HERE, "kernel_cpu", "driver.cpp")
+
+CASES = {
+    "Spherical_sym2_KO0_fd4": dict(coord=1, Nxx=(12, 8, 2), fd_order=4, size=7.5, variant=0),
+    "SinhSpherical_symnone_KO1_fd6": dict(coord=2, Nxx=(20, 10, 12), fd_order=6, size=64.0, variant=1),
+    "SinhSpherical_symnone_KO1_fd4": dict(coord=2, Nxx=(14, 8, 10), fd_order=4, size=64.0, variant=1),
+}
+
+
+def schedules(tag):
+    out = []
+    for f in sorted(os.listdir(GEN)):
+        m = re.match(rf"bssn_{tag}_(rhs|ricci)_(.+)\.hip$", f)
+        if m:
+            out.append((m.group(1), m.group(2)))
+    return out
+
+
+def build(tag, kind, cand, workdir):
+    src = os.path.join(GEN, f"bssn_{tag}_{kind}_{cand}.hip")
+    exe = os.path.join(workdir, f"{kind}_{cand}")
+    cmd = ["g++", "-std=c++17", "-O1", "-g", "-fsanitize=address", "-fno-omit-frame-pointer", "-w",
+           f"-I{SHIM}", f'-DKERNEL_FILE="{src}"', f"-DLAUNCH=launch_bssn_{kind}_{tag}_{cand}", DRIVER,
+           "-o", exe, "-lm"]
+    subprocess.run(cmd, check=True, capture_output=True)
+    return exe
+
+
+def run(exe, O, kind, y, aux, tile, workdir):
+    inp, outp = os.path.join(workdir, "in.bin"), os.path.join(workdir, "out.bin")
+    inv = [1.0 / (O.xx[d][1] - O.xx[d][0]) for d in range(3)]
+    nout = 24 if kind == "rhs" else 6
+    with open(inp, "wb") as f:
+        np.array([*O.Nx, O.NG, tile[0], tile[1], 24, 6 if kind == "rhs" else 0, nout], dtype=np.int32).tofile(f)
+        np.array(inv + [2.0, 0.3, 0.3, 1.0]).tofile(f)
+        y.tofile(f)
+        if kind == "rhs":
+            aux.tofile(f)
+        O.rfm0.tofile(f)
+        O.rfm1.tofile(f)
+    r = subprocess.run([exe, inp, outp], capture_output=True, text=True,
+                       env=dict(os.environ, ASAN_OPTIONS="detect_leaks=0"))
+    assert r.returncode == 0, r.stderr[-3000:]
+    return np.fromfile(outp).reshape(nout, *O.Nx[::-1])
+
+
+def interior(a, NG):
+    return a[:, NG:-NG, NG:-NG, NG:-NG]
+
+
+def rel_err(a, b):
+    num = np.abs(a - b).reshape(a.shape[0], -1).max(axis=1)
+    den = np.abs(b).reshape(b.shape[0], -1).max(axis=1)
+    return num / np.where(den > 0, den, 1.0)
+
+
+def _check(oracle_lib, tag, cands, tiles):
+    c = CASES[tag]
+    O = oracle_lib.OracleGrid(c["coord"], c["Nxx"], c["fd_order"], c["size"], 0.2, variant=c["variant"])
+    y = ID.brill_lindquist(O.coord, O.xx, O.g.p0, O.g.p1)
+    y = np.ascontiguousarray(ID.smooth_perturbation(y, O.xx, 3))
+    O.apply_bcs_outerextrap_and_inner(y)
+    aux = O.ricci(y)
+    ref = {"ricci": aux, "rhs": O.rhs(y, aux, O.params())}
+    wd = tempfile.mkdtemp(prefix="kcpu_")
+    try:
+        with ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
+            exes = list(ex.map(lambda kc: build(tag, kc[0], kc[1], wd), cands))
+        for (kind, cand), exe in zip(cands, exes):
+            for tile in tiles:
+                out = run(exe, O, kind, y, aux, tile, wd)
+                er = rel_err(interior(out, O.NG), interior(ref[kind], O.NG))
+                assert er.max() < 1e-12, (kind, cand, tile, er)
+    finally:
+        shutil.rmtree(wd, ignore_errors=True)
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
+def test_spherical_fd4_schedules_on_cpu(oracle_lib):
+    _check(oracle_lib, "Spherical_sym2_KO0_fd4", schedules("Spherical_sym2_KO0_fd4")[:2], [(64, 4), (32, 8)])
+
+
+@pytest.mark.slow
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
+@pytest.mark.parametrize("tag", list(CASES))
+def test_every_schedule_on_cpu(oracle_lib, tag):
+    _check(oracle_lib, tag, schedules(tag), [(64, 4), (32, 8)])
