@@ -8,7 +8,11 @@ data with masses 0.6 and 0.4 at (0.3,0.2,0.5) and (-0.3,-0.2,-0.5).
 value = interior points x timed RK4 steps / seconds of MoL_step_forward_in_time,
 with all fields resident in HBM.  Prints one JSON line.
 
-    python bench.py [--gpus 1] [--steps 100] [--warmup 5] [--config c3|c2] [--no-cpu]
+    python bench.py [--gpus 1] [--steps 100] [--warmup 5] [--config c3|c2] [--no-cpu] [--no-batched]
+
+The line also carries "batched": the 64-member ensemble (seeded masses, separations and
+grids) through the batched path against the same members stepped one after another
+with plain per-patch launches.
     python bench.py --copy-calibration 512   (plain 512 MiB device copy x3, for
                                               scaling the FETCH_SIZE/WRITE_SIZE counters)
 """
@@ -44,6 +48,98 @@ MEASURED_TRAFFIC = {
         "write_bytes": (152064 + 96768 + 387072) * 1024.0,
     },
 }
+
+
+# Batched ensemble: 64 independent head-on Brill-Lindquist evolutions on Spherical grids drawn
+# uniformly from four sizes, masses m1, m2 ~ U[0.3, 0.7], separation d ~ U[0.6, 2.0] along z
+# (punctures at z = +-d/2), rmax = 10, 4th-order FD, RK4, each member at its own CFL timestep.
+C4_GRIDS = [(48, 8, 2), (72, 12, 2), (96, 16, 4), (128, 24, 8)]
+C4_SEED = 20261202
+
+
+def c4_members(n: int = 64, seed: int = C4_SEED):
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    m1 = rng.uniform(0.3, 0.7, n)
+    m2 = rng.uniform(0.3, 0.7, n)
+    d = rng.uniform(0.6, 2.0, n)
+    g = rng.integers(0, len(C4_GRIDS), n)
+    return [dict(Nxx=C4_GRIDS[g[i]], m1=float(m1[i]), m2=float(m2[i]), d=float(d[i])) for i in range(n)]
+
+
+def bench_batched(steps: int, warmup: int, n: int = 64) -> dict:
+    """Ensemble throughput (sum over members of interior points x steps / s) through the
+    batched path, against the same members advanced one after another with plain
+    per-patch launches; both end states are compared member by member."""
+    import numpy as np
+    import torch
+
+    from nrpy_amd import autotune
+    from nrpy_amd import initial_data as ID
+    from nrpy_amd.runtime import Ensemble, MoLState, Patch, PatchConfig
+
+    mem = c4_members(n)
+    cfgs = [PatchConfig(Nxx=m["Nxx"], grid_physical_size=10.0, eta=2.0) for m in mem]
+    E = Ensemble(cfgs)
+    tuned = autotune.apply_tuned_ensemble(E)
+    y0 = []
+    for P, st, m in zip(E.members, E.states, mem):
+        y = ID.brill_lindquist(1, P.xx(), 10.0, 0.2, (m["m1"], m["m2"]), (0, 0, m["d"] / 2), (0, 0, -m["d"] / 2))
+        yd = torch.from_numpy(np.ascontiguousarray(y)).cuda()
+        P.apply_bcs_outerextrap_and_inner(yd)
+        y0.append(yd)
+        st.y_n.copy_(yd)
+    seq = []
+    for cfg, yd in zip(cfgs, y0):
+        Q = Patch(cfg, tuned=False)
+        Q.select_schedule(0, E.schedules(0)[E.choice[0]])
+        Q.select_schedule(1, E.schedules(1)[E.choice[1]])
+        Q.set_tile(*E.tile)
+        seq.append(MoLState(Q, yd.clone()))
+    npts = sum(c.Nxx[0] * c.Nxx[1] * c.Nxx[2] for c in cfgs)
+    stream = torch.cuda.current_stream()
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return max(time.perf_counter() - t0, e0.elapsed_time(e1) / 1e3)
+
+    if warmup:
+        E.steps(warmup)
+        for st, dt in zip(seq, E.dts):
+            st.steps(warmup, dt, use_graph=False)
+    t_batched = timed(lambda: E.steps(steps))
+
+    def sequential():
+        for _ in range(steps):
+            for st, dt in zip(seq, E.dts):
+                st.step(dt)
+
+    t_seq = timed(sequential)
+    err = 0.0
+    for st, sq in zip(E.states, seq):
+        a, b = st.y_n.cpu().numpy(), sq.y_n.cpu().numpy()
+        den = np.abs(b).reshape(24, -1).max(axis=1)
+        err = max(err, float((np.abs(a - b).reshape(24, -1).max(axis=1) / np.where(den > 0, den, 1)).max()))
+    return {
+        "workload": f"bssn_ensemble_{n}_spherical_fd4_rk4 (seed {C4_SEED})",
+        "members": n,
+        "interior_points": npts,
+        "steps": steps,
+        "value": npts * steps / t_batched,
+        "unit": "gridpoint-RK4-steps/s",
+        "sequential_plain_value": npts * steps / t_seq,
+        "vs_sequential_plain": t_seq / t_batched,
+        "max_rel_diff_vs_sequential": err,
+        "schedule": {"rhs": E.schedules(0)[E.choice[0]], "ricci": E.schedules(1)[E.choice[1]],
+                     "tile": list(E.tile), "max_workgroups": E.max_wg, "tuned": tuned is not None},
+    }
 
 
 def cpu_baseline(c: dict, steps: int = 1) -> dict:
@@ -83,6 +179,8 @@ def main() -> None:
     ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-ensemble comparison")
+    ap.add_argument("--batched-steps", type=int, default=200)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB three times, then exit")
     args = ap.parse_args()
@@ -207,6 +305,8 @@ def main() -> None:
             "ricci_tflops": flops_ricci * npts / (ricci_ms * 1e-3) / 1e12,
         },
     }
+    if not args.no_batched:
+        result["batched"] = bench_batched(args.batched_steps, 2)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(c)
     print(json.dumps(result))

@@ -183,6 +183,31 @@ int bhah_hip_wave_apply_bcs(bhah_patch *patch, double *gfs);
 int bhah_hip_diagnostics_constraints(bhah_patch *patch, const double *auxevol_gfs, const double *in_gfs, double *diag,
                                      double *l2_H);
 
+/* Batched ensembles: many independent BSSN patches of one kernel build, with uneven
+ * grids, advanced together.  No single reference counterpart: it batches
+ * MoL_step_forward_in_time (MoLtimestepping/MoL_step_forward_in_time.py:39-507) over
+ * independent evolutions.  Each stage of a substep is one persistent launch over the
+ * tile list of all members; bhah_hip_ensemble_step_n captures a whole step into a
+ * hipGraph.  Members must share coordinate system, FD order, symmetry axes and KO
+ * (spherical-like coordinates, radiation outer BCs). */
+typedef struct bhah_ensemble bhah_ensemble;
+int bhah_hip_ensemble_create(const bhah_patch_desc *descs, int n, bhah_ensemble **out);
+int bhah_hip_ensemble_destroy(bhah_ensemble *ens);
+int bhah_hip_ensemble_size(const bhah_ensemble *ens);
+/* The member's patch (grid, boundary structs, dt); owned by the ensemble. */
+int bhah_hip_ensemble_member(bhah_ensemble *ens, int m, bhah_patch **patch);
+/* Bind each member's five MoL arrays (n entries, all allocated, auxevol included) and
+ * timestep (dt == NULL: each member's CFL timestep). */
+int bhah_hip_ensemble_bind(bhah_ensemble *ens, const bhah_MoL_gridfunctions *gfs, const double *dt);
+int bhah_hip_ensemble_set_stream(bhah_ensemble *ens, void *hip_stream);
+int bhah_hip_ensemble_select_schedule(bhah_ensemble *ens, int kind, int index);
+/* Interior tile (tile_x * tile_y = 256) and the persistent grid size
+ * (max_workgroups; 0 = one workgroup per tile). */
+int bhah_hip_ensemble_set_launch(bhah_ensemble *ens, int tile_x, int tile_y, int max_workgroups);
+/* nsteps RK4 steps of every member, each with its own dt. */
+int bhah_hip_ensemble_step_n(bhah_ensemble *ens, int nsteps, int use_graph);
+int bhah_hip_ensemble_synchronize(bhah_ensemble *ens);
+
 #ifdef __cplusplus
 }
 #endif

@@ -8,6 +8,7 @@ stores the fastest choice in ``nrpy_amd/autotune/<build>_<N0>x<N1>x<N2>.json``.
 ``Patch`` applies a stored choice automatically (see ``apply_tuned``).
 
     python -m nrpy_amd.autotune --config c3
+    python -m nrpy_amd.autotune --config c4     (batched ensemble: schedules and launch shape)
 """
 
 from __future__ import annotations
@@ -138,6 +139,87 @@ def tune(cfg, reps: int = 5, write: bool = True, log=print) -> Dict:
     return out
 
 
+ENSEMBLE_LAUNCHES = [(64, 4, 0), (256, 1, 0), (128, 2, 0), (64, 4, 1024), (256, 1, 1024), (64, 4, 512),
+                     (256, 1, 512)]
+
+
+def ensemble_table_path(E) -> str:
+    n = len(E.members)
+    cells = sum(P.cfg.Nxx[0] * P.cfg.Nxx[1] * P.cfg.Nxx[2] for P in E.members)
+    return os.path.join(TABLE_DIR, f"ensemble_{build_tag(E.cfgs[0])}_{n}x{cells}.json")
+
+
+def apply_tuned_ensemble(E) -> Optional[Dict]:
+    """Select the stored schedules and launch shape for an ensemble (exact member count and
+    total interior points, else the nearest stored ensemble of the same build)."""
+    from .runtime import KERNEL_RHS, KERNEL_RICCI
+
+    p = ensemble_table_path(E)
+    d = None
+    if os.path.exists(p):
+        with open(p) as f:
+            d = json.load(f)
+    elif os.path.isdir(TABLE_DIR):
+        pre = f"ensemble_{build_tag(E.cfgs[0])}_"
+        cands = sorted(fn for fn in os.listdir(TABLE_DIR) if fn.startswith(pre))
+        if cands:
+            with open(os.path.join(TABLE_DIR, cands[-1])) as f:
+                d = json.load(f)
+    if not d:
+        return None
+    if d["rhs"] in E.schedules(KERNEL_RHS):
+        E.select_schedule(KERNEL_RHS, d["rhs"])
+    if d["ricci"] in E.schedules(KERNEL_RICCI):
+        E.select_schedule(KERNEL_RICCI, d["ricci"])
+    E.set_launch(*d["launch"])
+    return d
+
+
+def tune_ensemble(E, steps: int = 3, write: bool = True, log=print) -> Dict:
+    """Time every batched schedule pair, then every launch shape, on whole RK4 steps."""
+    import torch
+
+    from .runtime import KERNEL_RHS, KERNEL_RICCI
+
+    saved = [st.y_n.clone() for st in E.states]
+    stream = torch.cuda.current_stream()
+
+    def run():
+        for st, y in zip(E.states, saved):
+            st.y_n.copy_(y)
+        E.steps(steps)
+
+    timings: Dict[str, float] = {}
+    best = {}
+    E.set_launch(64, 4, 0)
+    for kind, label in ((KERNEL_RICCI, "ricci"), (KERNEL_RHS, "rhs")):
+        res = {}
+        for name in E.schedules(kind):
+            E.select_schedule(kind, name)
+            res[name] = _time(run, stream, 2)
+            timings[f"{label}:{name}"] = res[name]
+            log(f"ensemble {label} {name}: {res[name]:.3f} ms / {steps} steps")
+        best[label] = min(res, key=res.get)
+        E.select_schedule(kind, best[label])
+    lres = {}
+    for L in ENSEMBLE_LAUNCHES:
+        E.set_launch(*L)
+        lres[L] = _time(run, stream, 2)
+        timings[f"launch:{L[0]}x{L[1]}:{L[2]}"] = lres[L]
+        log(f"ensemble launch {L}: {lres[L]:.3f} ms / {steps} steps")
+    launch = min(lres, key=lres.get)
+    E.set_launch(*launch)
+    for st, y in zip(E.states, saved):
+        st.y_n.copy_(y)
+    out = {"build": build_tag(E.cfgs[0]), "members": len(E.members), "rhs": best["rhs"], "ricci": best["ricci"],
+           "launch": list(launch), "timings_ms": timings, "device": torch.cuda.get_device_name(0)}
+    if write:
+        os.makedirs(TABLE_DIR, exist_ok=True)
+        with open(ensemble_table_path(E), "w") as f:
+            json.dump(out, f, indent=1, sort_keys=True)
+    return out
+
+
 def main() -> None:
     import sys
 
@@ -150,6 +232,27 @@ def main() -> None:
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--out", default=None, help="also copy the table here")
     a = ap.parse_args()
+    if a.config == "c4":
+        import numpy as np
+        import torch
+
+        from bench import c4_members
+        from . import initial_data as ID
+        from .runtime import Ensemble, PatchConfig
+
+        mem = c4_members()
+        E = Ensemble([PatchConfig(Nxx=m["Nxx"], grid_physical_size=10.0, eta=2.0) for m in mem])
+        for P, st, m in zip(E.members, E.states, mem):
+            y = ID.brill_lindquist(1, P.xx(), 10.0, 0.2, (m["m1"], m["m2"]), (0, 0, m["d"] / 2), (0, 0, -m["d"] / 2))
+            st.y_n.copy_(torch.from_numpy(np.ascontiguousarray(y)).cuda())
+            P.apply_bcs_outerextrap_and_inner(st.y_n)
+        out = tune_ensemble(E)
+        if a.out:
+            os.makedirs(os.path.dirname(a.out), exist_ok=True)
+            with open(a.out, "w") as f:
+                json.dump(out, f, indent=1, sort_keys=True)
+        print(json.dumps({k: out[k] for k in ("build", "members", "rhs", "ricci", "launch")}))
+        return
     c = CONFIGS[a.config]
     cfg = PatchConfig(coord=c["coord"], Nxx=c["Nxx"], fd_order=c["fd_order"], symmetry_axes=c["sym"],
                       enable_KO=c["KO"], grid_physical_size=c["size"], SINHW=c["sinhw"], eta=2.0)

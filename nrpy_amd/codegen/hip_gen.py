@@ -534,15 +534,8 @@ def _schedule_prefetch(items: List[tuple], budget: int, ahead: int) -> List[tupl
     return out
 
 
-KPARAMS = r"""
-struct bhah_kparams {
-  int Nx0, Nx1, Nx2, NG;
-  long Ntot, s1, s2;
-  double invdxx0, invdxx1, invdxx2;
-  double eta, KO_gauge, KO_nongauge, wavespeed;
-  double dt;
-  int substep;
-};
+KPARAMS = """
+#include "../bhah_device_types.h"
 """
 
 
@@ -560,6 +553,46 @@ XCD_REMAP = """  // Workgroups are dispatched round-robin over the 8 XCDs (each 
       bz = l / (gx * gy);
     }
   }
+"""
+
+
+def _prologue_at(interior: bool) -> str:
+    """Prologue of a per-tile device function: tile coordinates come in as bx, by, bz."""
+    lo = "p.NG" if interior else "0"
+    return f"""  const int i0 = {lo} + (int)(bx * blockDim.x + threadIdx.x);
+  const int i1 = {lo} + (int)(by * blockDim.y + threadIdx.y);
+  const int i2 = {lo} + bz;
+  if (i0 >= p.Nx0 - {lo} || i1 >= p.Nx1 - {lo} || i2 >= p.Nx2 - {lo}) return;
+  const long s1 = p.s1, s2 = p.s2, Ntot = p.Ntot;
+  const long idx = i0 + s1 * i1 + s2 * i2;
+"""
+
+
+ENS_SIG = ("dim3 g, dim3 b, hipStream_t s, const bhah_member *mem, const bhah_item *items, int nitems, "
+           "int in_sel, int aux_sel, int out_sel")
+
+
+def _ensemble_kernel(kname: str, upw: str, body: str, lb: str) -> str:
+    """Persistent batched form of one interior kernel: a fixed grid of workgroups
+    walks the ensemble's tile list; each tile runs the same per-point code on its
+    member's arrays (selected by index from the member table)."""
+    return f"""
+static __device__ __forceinline__ void {kname}_tile(const double *__restrict__ in, const double *__restrict__ aux,
+        double *__restrict__ out, const double *__restrict__ rfm0, const double *__restrict__ rfm1,
+        const bhah_kparams p, const int bx, const int by, const int bz) {{
+{_prologue_at(True)}
+{upw}
+{body}
+}}
+extern "C" __global__ void __launch_bounds__({lb})
+{kname}(const bhah_member *__restrict__ mem, const bhah_item *__restrict__ items, const int nitems,
+        const int in_sel, const int aux_sel, const int out_sel) {{
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {{
+    const bhah_item w = items[it];
+    const bhah_member *M = &mem[w.m];
+    {kname}_tile(M->arr[in_sel], M->arr[aux_sel], M->arr[out_sel], M->rfm0, M->rfm1, M->kp, w.bx, w.by, w.bz);
+  }}
+}}
 """
 
 
@@ -696,7 +729,7 @@ def _subspec(spec: specs.Spec, name: str, gfs: List[str]) -> specs.Spec:
                       defs=spec.defs)
 
 
-def candidate_tu(v: specs.Variant, fd_order: int, kind: str, cand: str) -> Tuple[str, int]:
+def candidate_tu(v: specs.Variant, fd_order: int, kind: str, cand: str, ensemble: bool = False) -> Tuple[str, int]:
     """One candidate schedule: group kernels plus a launcher running them in order.
 
     ``cand`` is ``<partition>_<launch bounds>[_<mode>]``; mode ``x`` (default) CSEs the
@@ -716,6 +749,7 @@ def candidate_tu(v: specs.Variant, fd_order: int, kind: str, cand: str) -> Tuple
     xcd = "xcd" in opts
     rats: Dict[str, str] = {}
     kernels, calls, total = [], [], 0
+    ens_kernels, ens_calls = [], []
     if kind == "rhs":
         spec = specs.bssn_rhs_spec(v, staged=staged)
         groups = RHS_PARTITIONS[part]
@@ -748,6 +782,10 @@ extern "C" __global__ void __launch_bounds__({LAUNCH_BOUNDS[lb]})
 }}
 """)
         calls.append(f"  hipLaunchKernelGGL({kname}, g, b, 0, s, in, aux, out, rfm0, rfm1, p);")
+        if ensemble:
+            ename = f"bssn_ens_{kind}_{tag}_{cand}_g{k}"
+            ens_kernels.append(_ensemble_kernel(ename, upw, body, LAUNCH_BOUNDS[lb]))
+            ens_calls.append(f"  hipLaunchKernelGGL({ename}, g, b, 0, s, mem, items, nitems, in_sel, aux_sel, out_sel);")
     consts = "\n".join(f"#define {k} ({v})" for k, v in sorted(rats.items()))
     undefs = "\n".join(f"#undef {k}" for k in sorted(rats))
     launcher = f"""
@@ -756,12 +794,20 @@ extern "C" hipError_t launch_bssn_{kind}_{tag}_{cand}({CAND_SIG}) {{
   return hipGetLastError();
 }}
 """
+    if ensemble:
+        launcher += f"""
+extern "C" hipError_t launch_bssn_ens_{kind}_{tag}_{cand}({ENS_SIG}) {{
+{chr(10).join(ens_calls)}
+  return hipGetLastError();
+}}
+"""
     head = f"// ---- {kind} candidate {cand}: {len(groups)} kernel(s); {v.coord}, fd_order={fd_order}\n"
-    return head + consts + "\n" + "".join(kernels) + undefs + "\n" + launcher, total
+    return head + consts + "\n" + "".join(kernels) + "".join(ens_kernels) + undefs + "\n" + launcher, total
 
 
-def bssn_kernels(v: specs.Variant, fd_order: int) -> str:
-    """Kernels shared by every schedule: RK4+det pass, det enforcement, constraints."""
+def bssn_kernels(v: specs.Variant, fd_order: int, ensemble: bool = False) -> str:
+    """Kernels shared by every schedule: RK4+det pass, det enforcement, constraints
+    (and, for ensemble builds, the batched RK4+det pass)."""
     tag = f"{v.tag}_fd{fd_order}"
     rats: Dict[str, str] = {}
     ngf = len(bssn.EVOL_GFS)
@@ -783,6 +829,38 @@ bssn_rk_det_{tag}(const double *__restrict__ kin, const double *y_n, double *__r
 {det}
 #pragma unroll
   for (int g = 0; g < NGF; g++) out[g * Ntot + idx] = nxt[g];
+}}
+"""
+    ens_rk = ""
+    if ensemble:
+        ens_rk = f"""
+static __device__ __forceinline__ void bssn_ens_rk_det_{tag}_tile(const double *__restrict__ kin, const double *y_n,
+        double *__restrict__ y_run, double *out, const double *__restrict__ rfm0, const double *__restrict__ rfm1,
+        const bhah_kparams p, const int bx, const int by, const int bz) {{
+{_prologue_at(False)}
+  constexpr int NGF = {ngf};
+  double kk[NGF], ctr[NGF];
+#pragma unroll
+  for (int g = 0; g < NGF; g++) {{
+    kk[g] = kin[g * Ntot + idx];
+    ctr[g] = (p.substep == 1) ? y_n[g * Ntot + idx] : 0.0;
+  }}
+{RK_EPILOGUE}
+{det}
+#pragma unroll
+  for (int g = 0; g < NGF; g++) out[g * Ntot + idx] = nxt[g];
+}}
+extern "C" __global__ void __launch_bounds__(256)
+bssn_ens_rk_det_{tag}(const bhah_member *__restrict__ mem, const bhah_item *__restrict__ items, const int nitems,
+                      const int k_sel, const int out_sel, const int substep) {{
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {{
+    const bhah_item w = items[it];
+    const bhah_member *M = &mem[w.m];
+    bhah_kparams p = M->kp;
+    p.substep = substep;
+    bssn_ens_rk_det_{tag}_tile(M->arr[k_sel], M->arr[0], M->arr[1], M->arr[out_sel], M->rfm0, M->rfm1, p, w.bx, w.by,
+                               w.bz);
+  }}
 }}
 """
     # ---- det enforcement over all points (standalone ABI call)
@@ -833,7 +911,14 @@ extern "C" hipError_t launch_bssn_detgamma_{tag}(dim3 g, dim3 b, hipStream_t s, 
   return hipGetLastError();
 }}
 """
-    return "\n".join(head) + "\n" + consts + "\n" + rkdet + det_all + cons + undefs + "\n" + launch
+    if ensemble:
+        launch += f"""extern "C" hipError_t launch_bssn_ens_rk_det_{tag}(dim3 g, dim3 b, hipStream_t s, const bhah_member *mem,
+    const bhah_item *items, int nitems, int k_sel, int out_sel, int substep) {{
+  hipLaunchKernelGGL(bssn_ens_rk_det_{tag}, g, b, 0, s, mem, items, nitems, k_sel, out_sel, substep);
+  return hipGetLastError();
+}}
+"""
+    return "\n".join(head) + "\n" + consts + "\n" + rkdet + ens_rk + det_all + cons + undefs + "\n" + launch
 
 
 def wave_kernels(fd_order: int) -> str:
@@ -882,10 +967,11 @@ extern "C" hipError_t launch_wave_rhs_rk_{tag}(dim3 g, dim3 b, hipStream_t s, co
 """
 
 
+# (variant, fd order, candidate schedules, batched-ensemble kernels too)
 BSSN_BUILDS = [
-    (specs.VARIANTS[0], 4, SMALL_CANDIDATES),
-    (specs.VARIANTS[1], 6, FULL_CANDIDATES),
-    (specs.VARIANTS[1], 4, SMALL_CANDIDATES),
+    (specs.VARIANTS[0], 4, SMALL_CANDIDATES, True),
+    (specs.VARIANTS[1], 6, FULL_CANDIDATES, False),
+    (specs.VARIANTS[1], 4, SMALL_CANDIDATES, False),
 ]
 WAVE_BUILDS = [4]
 COORD_ENUM = {"Cartesian": 0, "Spherical": 1, "SinhSpherical": 2}
@@ -906,32 +992,42 @@ def generate(outdir: str) -> List[str]:
     hdr = "// Generated by nrpy_amd/codegen/hip_gen.py -- do not edit.\n#include <hip/hip_runtime.h>\n#include <math.h>\n" + KPARAMS
     meta = {"flops_per_point": {}, "candidates": {}}
     table_decl, table_rows = [], []
-    for v, order, cands in BSSN_BUILDS:
+    for v, order, cands, ens in BSSN_BUILDS:
         tag = kernel_name_tag(v, order)
         path = os.path.join(outdir, f"bssn_{tag}.hip")
         with open(path, "w", encoding="utf-8") as f:
-            f.write(hdr + bssn_kernels(v, order))
+            f.write(hdr + bssn_kernels(v, order, ensemble=ens))
         paths.append(path)
         arrays = {}
         for kind in ("rhs", "ricci"):
             rows = []
             for cand in cands[kind]:
-                src, fl = candidate_tu(v, order, kind, cand)
+                src, fl = candidate_tu(v, order, kind, cand, ensemble=ens)
                 path = os.path.join(outdir, f"bssn_{tag}_{kind}_{cand}.hip")
                 with open(path, "w", encoding="utf-8") as f:
                     f.write(hdr + src)
                 paths.append(path)
                 meta["candidates"][f"{kind}_{tag}_{cand}"] = fl
                 table_decl.append(f'extern "C" hipError_t launch_bssn_{kind}_{tag}_{cand}({CAND_SIG});')
-                rows.append(f'{{"{cand}", launch_bssn_{kind}_{tag}_{cand}}}')
+                if ens:
+                    table_decl.append(f'extern "C" hipError_t launch_bssn_ens_{kind}_{tag}_{cand}({ENS_SIG});')
+                    rows.append(f'{{"{cand}", launch_bssn_{kind}_{tag}_{cand}, launch_bssn_ens_{kind}_{tag}_{cand}}}')
+                else:
+                    rows.append(f'{{"{cand}", launch_bssn_{kind}_{tag}_{cand}, nullptr}}')
             arrays[kind] = f"kCand_{kind}_{tag}"
             table_decl.append(f"static const BssnCandidate {arrays[kind]}[] = {{{', '.join(rows)}}};")
         sym_mask = sum(1 << int(c) for c in v.symmetry_axes)
         table_rows.append(
             f"    {{{COORD_ENUM[v.coord]}, {sym_mask}, {int(v.enable_KO)}, {order}, \"{tag}\", "
             f"{arrays['rhs']}, {len(cands['rhs'])}, {arrays['ricci']}, {len(cands['ricci'])}, "
-            f"launch_bssn_rk_det_{tag}, launch_bssn_detgamma_{tag}, launch_bssn_constraints_{tag}}},"
+            f"launch_bssn_rk_det_{tag}, launch_bssn_detgamma_{tag}, launch_bssn_constraints_{tag}, "
+            f"{f'launch_bssn_ens_rk_det_{tag}' if ens else 'nullptr'}}},"
         )
+        if ens:
+            table_decl.append(
+                f'extern "C" hipError_t launch_bssn_ens_rk_det_{tag}(dim3, dim3, hipStream_t, const bhah_member *, '
+                f"const bhah_item *, int, int, int, int);"
+            )
         table_decl.append(
             f'extern "C" hipError_t launch_bssn_rk_det_{tag}(dim3, dim3, hipStream_t, const double *, const double *, '
             f"double *, double *, const double *, const double *, bhah_kparams);"

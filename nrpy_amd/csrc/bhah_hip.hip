@@ -16,14 +16,7 @@
 
 #include "../../include/bhah_hip.h"
 
-struct bhah_kparams {
-  int Nx0, Nx1, Nx2, NG;
-  long Ntot, s1, s2;
-  double invdxx0, invdxx1, invdxx2;
-  double eta, KO_gauge, KO_nongauge, wavespeed;
-  double dt;
-  int substep;
-};
+#include "bhah_device_types.h"
 
 // ---------------------------------------------------------------------------
 // Generated kernels: per (coordinate, symmetry, KO, fd order) build, a set of
@@ -31,9 +24,11 @@ struct bhah_kparams {
 // budgets), plus the RK4+det pass, det enforcement and constraints.
 typedef hipError_t (*bhah_cand_fn)(dim3, dim3, hipStream_t, const double *, const double *, double *,
                                    const double *, const double *, bhah_kparams);
+typedef hipError_t (*bhah_ens_fn)(dim3, dim3, hipStream_t, const bhah_member *, const bhah_item *, int, int, int, int);
 struct BssnCandidate {
   const char *name;
   bhah_cand_fn launch;
+  bhah_ens_fn ens;  // batched-ensemble form, where the build has one
 };
 struct BssnKernels {
   int coord, sym_axes, KO, fd_order;
@@ -47,6 +42,7 @@ struct BssnKernels {
   hipError_t (*detgamma)(dim3, dim3, hipStream_t, double *, const double *, const double *, bhah_kparams);
   hipError_t (*constraints)(dim3, dim3, hipStream_t, const double *, const double *, double *, const double *,
                             const double *, bhah_kparams);
+  hipError_t (*ens_rk_det)(dim3, dim3, hipStream_t, const bhah_member *, const bhah_item *, int, int, int, int);
 };
 #include "gen/kernel_table.inc"
 
@@ -190,16 +186,11 @@ __device__ __forceinline__ double partial_r_f(const RadStencil &st, int R, const
   return dx0dr * (acc * invdxx0);
 }
 
-__global__ void __launch_bounds__(256)
-bc_radiation_kernel(const int16_t *__restrict__ pts, int npts, int ngfs, const double *__restrict__ gfs,
-                    double *__restrict__ rhs, const double *__restrict__ xx0, const double *__restrict__ wavespeed,
-                    const double *__restrict__ finf, RadStencil st, int R, int coord, double AMPL, double SINHW,
-                    int Nx0, long s1, long s2, long Ntot, double invdxx0) {
+__device__ __forceinline__ void rad_point(const int16_t *q, int gf, const double *__restrict__ gfs,
+                                          double *__restrict__ rhs, const double *__restrict__ xx0, double c,
+                                          double f_inf, const RadStencil &st, int R, int coord, double AMPL,
+                                          double SINHW, int Nx0, long s1, long s2, long Ntot, double invdxx0) {
 #pragma clang fp contract(off)
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)npts * ngfs) return;
-  const int p = (int)(t % npts), gf = (int)(t / npts);
-  const int16_t *q = &pts[6 * p];
   const int i0 = q[0], i1 = q[1], i2 = q[2];
   const int j0 = i0 + q[3], j1 = i1 + q[4], j2 = i2 + q[5];
   double r, dxr, r_int, dxr_int;
@@ -210,13 +201,61 @@ bc_radiation_kernel(const int16_t *__restrict__ pts, int npts, int ngfs, const d
   const double *f = &gfs[gf * Ntot];
   const double dfr = partial_r_f(st, R, f, Nx0, idx, i0, q[3], dxr, invdxx0);
   const double dfr_int = partial_r_f(st, R, f, Nx0, idx_int, j0, q[3], dxr_int, invdxx0);
-  const double c = wavespeed[gf], f_inf = finf[gf];
   const double dtf_int = rhs[gf * Ntot + idx_int];
   const double dtf_int_out = -c * (dfr_int + (f[idx_int] - f_inf) / r_int);
   const double k = r_int * r_int * r_int * (dtf_int - dtf_int_out);
   const double rinv = 1.0 / r;
   const double dtf_out = -c * (dfr + (f[idx] - f_inf) * rinv);
   rhs[gf * Ntot + idx] = dtf_out + k * rinv * rinv * rinv;
+}
+
+__global__ void __launch_bounds__(256)
+bc_radiation_kernel(const int16_t *__restrict__ pts, int npts, int ngfs, const double *__restrict__ gfs,
+                    double *__restrict__ rhs, const double *__restrict__ xx0, const double *__restrict__ wavespeed,
+                    const double *__restrict__ finf, RadStencil st, int R, int coord, double AMPL, double SINHW,
+                    int Nx0, long s1, long s2, long Ntot, double invdxx0) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)npts * ngfs) return;
+  const int p = (int)(t % npts), gf = (int)(t / npts);
+  rad_point(&pts[6 * p], gf, gfs, rhs, xx0, wavespeed[gf], finf[gf], st, R, coord, AMPL, SINHW, Nx0, s1, s2, Ntot,
+            invdxx0);
+}
+
+// Batched forms over an ensemble: one record per (member, boundary point), one
+// thread per (record, gridfunction); member arrays are picked by index.
+struct ens_outer_pt {
+  int m;
+  int16_t q[6];  // i0, i1, i2, face normal
+};
+struct ens_inner_pt {
+  int m, dst, src;
+  unsigned mask;
+};
+
+__global__ void __launch_bounds__(256)
+ens_bc_radiation_kernel(const bhah_member *__restrict__ mem, const ens_outer_pt *__restrict__ pts, int npts,
+                        int ngfs, int in_sel, int rhs_sel, const double *__restrict__ wavespeed,
+                        const double *__restrict__ finf, RadStencil st, int R) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)npts * ngfs) return;
+  const int p = (int)(t % npts), gf = (int)(t / npts);
+  const ens_outer_pt e = pts[p];
+  const bhah_member *M = &mem[e.m];
+  rad_point(e.q, gf, M->arr[in_sel], M->arr[rhs_sel], M->xx0, wavespeed[gf], finf[gf], st, R, M->coord, M->AMPL,
+            M->SINHW, M->kp.Nx0, M->kp.s1, M->kp.s2, M->kp.Ntot, M->kp.invdxx0);
+}
+
+__global__ void __launch_bounds__(256)
+ens_bc_inner_kernel(const bhah_member *__restrict__ mem, const ens_inner_pt *__restrict__ pts, int npts, int ngfs,
+                    const int *__restrict__ gf_parity, int sel) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)npts * ngfs) return;
+  const int p = (int)(t % npts), gf = (int)(t / npts);
+  const ens_inner_pt e = pts[p];
+  const bhah_member *M = &mem[e.m];
+  double *f = &M->arr[sel][gf * M->kp.Ntot];
+  const double v = f[e.src];
+  f[e.dst] = ((e.mask >> gf_parity[gf]) & 1u) ? -v : v;
 }
 
 // Quadratic extrapolation f0 = 3 f1 - 3 f2 + f3 along the face normal
@@ -967,6 +1006,321 @@ int bhah_hip_diagnostics_constraints(bhah_patch *P, const double *aux, const dou
     l2_H[1] = sqrt(red[1]);           // sqrt of the volume integral of H^2
     l2_H[2] = red[0];                 // proper volume
   }
+  return BHAH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batched ensembles: many independent BSSN patches of one kernel build (uneven
+// grids allowed) advanced together.  Every stage of an RK4 substep is one
+// persistent launch over the tile list of all members (a fixed grid of
+// workgroups walks the list), so a whole step is ~5 launches per substep
+// whatever the member count, and is captured once into a hipGraph.
+
+int bhah_hip_ensemble_destroy(bhah_ensemble *E);
+
+}  // extern "C"
+
+struct bhah_ensemble {
+  std::vector<bhah_patch *> P;
+  const BssnKernels *k = nullptr;
+  int NG = 0;
+  std::vector<bhah_member> hmem;
+  bhah_member *d_mem = nullptr;
+  bhah_item *d_int = nullptr, *d_all = nullptr;
+  int n_int = 0, n_all = 0;
+  ens_outer_pt *d_outer[8] = {};
+  int n_outer[8] = {};
+  ens_inner_pt *d_inner = nullptr;
+  int n_inner = 0;
+  double *d_wavespeed = nullptr, *d_finf = nullptr;
+  int *d_gf_parity = nullptr;
+  RadStencil st;
+  int R = 2;
+  int rhs_choice = 0, ricci_choice = 0;
+  int tile_x = 64, tile_y = 4, max_wg = 0;
+  bool bound = false;
+  hipStream_t stream = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+
+  void invalidate_graph() {
+    if (graph_exec) {
+      hipGraphExecDestroy(graph_exec);
+      graph_exec = nullptr;
+    }
+  }
+  dim3 grid_for(int nitems) const {
+    const int g = (max_wg > 0 && max_wg < nitems) ? max_wg : nitems;
+    return dim3(g > 0 ? g : 1);
+  }
+};
+
+namespace {
+
+// Tile lists: interior tiles (tile_x x tile_y threads over (i0, i1), one xx2 plane each)
+// and whole-grid tiles (64 x 4) for the RK pass, all members concatenated.
+int ensemble_build_items(bhah_ensemble *E) {
+  std::vector<bhah_item> in, all;
+  for (int m = 0; m < (int)E->P.size(); m++) {
+    const bhah_patch *P = E->P[m];
+    const int n0 = P->d.Nxx[0], n1 = P->d.Nxx[1], n2 = P->d.Nxx[2];
+    for (int bz = 0; bz < n2; bz++)
+      for (int by = 0; by < (n1 + E->tile_y - 1) / E->tile_y; by++)
+        for (int bx = 0; bx < (n0 + E->tile_x - 1) / E->tile_x; bx++) in.push_back({m, bx, by, bz});
+    for (int bz = 0; bz < P->Nx[2]; bz++)
+      for (int by = 0; by < (P->Nx[1] + 3) / 4; by++)
+        for (int bx = 0; bx < (P->Nx[0] + 63) / 64; bx++) all.push_back({m, bx, by, bz});
+  }
+  hipFree(E->d_int);
+  hipFree(E->d_all);
+  E->d_int = E->d_all = nullptr;
+  int rc;
+  if ((rc = upload(in, &E->d_int)) || (rc = upload(all, &E->d_all))) return rc;
+  E->n_int = (int)in.size();
+  E->n_all = (int)all.size();
+  return BHAH_OK;
+}
+
+int ensemble_enqueue_step(bhah_ensemble *E) {
+  const int in_s[4] = {0, 2, 3, 2}, k_s[4] = {2, 3, 2, 3};
+  const BssnCandidate &ric = E->k->ricci[E->ricci_choice], &rhs = E->k->rhs[E->rhs_choice];
+  const dim3 tb(E->tile_x, E->tile_y, 1), gi = E->grid_for(E->n_int);
+  for (int s = 1; s <= 4; s++) {
+    const int in = in_s[s - 1], k = k_s[s - 1], out = (s == 4) ? 0 : k;
+    HIP_TRY(ric.ens(gi, tb, E->stream, E->d_mem, E->d_int, E->n_int, in, 4, 4));
+    HIP_TRY(rhs.ens(gi, tb, E->stream, E->d_mem, E->d_int, E->n_int, in, 4, k));
+    for (int gz = 0; gz < E->NG; gz++) {
+      if (!E->n_outer[gz]) continue;
+      dim3 g, b;
+      launch_1d((long)E->n_outer[gz] * BHAH_NUM_EVOL_GFS_BSSN, g, b);
+      hipLaunchKernelGGL(ens_bc_radiation_kernel, g, b, 0, E->stream, E->d_mem, E->d_outer[gz], E->n_outer[gz],
+                         BHAH_NUM_EVOL_GFS_BSSN, in, k, E->d_wavespeed, E->d_finf, E->st, E->R);
+      HIP_TRY(hipGetLastError());
+    }
+    if (E->n_inner) {
+      dim3 g, b;
+      launch_1d((long)E->n_inner * BHAH_NUM_EVOL_GFS_BSSN, g, b);
+      hipLaunchKernelGGL(ens_bc_inner_kernel, g, b, 0, E->stream, E->d_mem, E->d_inner, E->n_inner,
+                         BHAH_NUM_EVOL_GFS_BSSN, E->d_gf_parity, k);
+      HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(E->k->ens_rk_det(E->grid_for(E->n_all), dim3(64, 4, 1), E->stream, E->d_mem, E->d_all, E->n_all, k, out, s));
+  }
+  return BHAH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bhah_hip_ensemble_create(const bhah_patch_desc *descs, int n, bhah_ensemble **out) {
+  if (!descs || !out || n < 1) return fail(BHAH_ERR_ARG, "bad arguments");
+  *out = nullptr;
+  bhah_ensemble *E = new bhah_ensemble();
+  int rc;
+  for (int m = 0; m < n; m++) {
+    const bhah_patch_desc &d = descs[m];
+    if (d.system != BHAH_SYSTEM_BSSN || d.outer_bc != BHAH_OUTER_BC_RADIATION || d.coord == BHAH_CARTESIAN) {
+      bhah_hip_ensemble_destroy(E);
+      return fail(BHAH_ERR_UNSUPPORTED, "ensembles take BSSN patches in spherical-like coordinates with radiation BCs");
+    }
+    bhah_patch *P = nullptr;
+    if ((rc = bhah_hip_patch_create(&d, &P))) {
+      bhah_hip_ensemble_destroy(E);
+      return rc;
+    }
+    E->P.push_back(P);
+    if (m == 0) {
+      E->k = P->k;
+      E->NG = P->NG;
+    } else if (P->k != E->k || d.radiation_bc_fd_order != descs[0].radiation_bc_fd_order) {
+      bhah_hip_ensemble_destroy(E);
+      return fail(BHAH_ERR_ARG, "member %d uses a different kernel build than member 0", m);
+    }
+  }
+  if (!E->k->ens_rk_det || !E->k->rhs[0].ens) {
+    bhah_hip_ensemble_destroy(E);
+    return fail(BHAH_ERR_UNSUPPORTED, "kernel build %s has no batched-ensemble kernels", E->k->tag);
+  }
+  E->R = (descs[0].radiation_bc_fd_order > 0 ? descs[0].radiation_bc_fd_order : 4) / 2;
+  if (E->R < 1 || E->R > 3) {
+    bhah_hip_ensemble_destroy(E);
+    return fail(BHAH_ERR_ARG, "radiation_bc_fd_order must be 2, 4 or 6");
+  }
+  rad_stencil(E->R, E->st);
+  // concatenated boundary records
+  std::vector<ens_inner_pt> inner;
+  std::vector<ens_outer_pt> outer[8];
+  for (int m = 0; m < n; m++) {
+    const bhah_patch *P = E->P[m];
+    for (size_t p = 0; p < P->inner_dst.size(); p++) {
+      unsigned mk = 0;
+      for (int t = 0; t < 10; t++)
+        if (P->inner_parity[10 * p + t] < 0) mk |= 1u << t;
+      inner.push_back({m, (int)P->inner_dst[p], (int)P->inner_src[p], mk});
+    }
+    for (int gz = 0; gz < E->NG; gz++)
+      for (int d = 0; d < 3; d++)
+        for (size_t q = 0; q < P->outer[gz][d].size(); q += 6) {
+          ens_outer_pt e;
+          e.m = m;
+          for (int j = 0; j < 6; j++) e.q[j] = P->outer[gz][d][q + j];
+          outer[gz].push_back(e);
+        }
+  }
+  std::vector<double> ws(24, 1.0), fi(24, 0.0);
+  ws[6] = sqrt(2.0);
+  fi[6] = 1.0;
+  fi[10] = 1.0;
+  std::vector<int> par(kBssnParity, kBssnParity + 24);
+  if ((rc = upload(inner, &E->d_inner)) || (rc = upload(ws, &E->d_wavespeed)) || (rc = upload(fi, &E->d_finf)) ||
+      (rc = upload(par, &E->d_gf_parity))) {
+    bhah_hip_ensemble_destroy(E);
+    return rc;
+  }
+  E->n_inner = (int)inner.size();
+  for (int gz = 0; gz < E->NG; gz++) {
+    if ((rc = upload(outer[gz], &E->d_outer[gz]))) {
+      bhah_hip_ensemble_destroy(E);
+      return rc;
+    }
+    E->n_outer[gz] = (int)outer[gz].size();
+  }
+  if ((rc = ensemble_build_items(E))) {
+    bhah_hip_ensemble_destroy(E);
+    return rc;
+  }
+  if (hipMalloc((void **)&E->d_mem, sizeof(bhah_member) * n) != hipSuccess) {
+    bhah_hip_ensemble_destroy(E);
+    return fail(BHAH_ERR_HIP, "hipMalloc failed");
+  }
+  E->hmem.resize(n);
+  *out = E;
+  return BHAH_OK;
+}
+
+int bhah_hip_ensemble_destroy(bhah_ensemble *E) {
+  if (!E) return BHAH_OK;
+  if (E->graph_exec) hipGraphExecDestroy(E->graph_exec);
+  for (bhah_patch *P : E->P) bhah_hip_patch_destroy(P);
+  hipFree(E->d_mem);
+  hipFree(E->d_int);
+  hipFree(E->d_all);
+  for (int gz = 0; gz < 8; gz++) hipFree(E->d_outer[gz]);
+  hipFree(E->d_inner);
+  hipFree(E->d_wavespeed);
+  hipFree(E->d_finf);
+  hipFree(E->d_gf_parity);
+  delete E;
+  return BHAH_OK;
+}
+
+int bhah_hip_ensemble_size(const bhah_ensemble *E) {
+  if (!E) return fail(BHAH_ERR_ARG, "NULL ensemble");
+  return (int)E->P.size();
+}
+
+int bhah_hip_ensemble_member(bhah_ensemble *E, int m, bhah_patch **patch) {
+  if (!E || !patch || m < 0 || m >= (int)E->P.size()) return fail(BHAH_ERR_ARG, "bad arguments");
+  *patch = E->P[m];
+  return BHAH_OK;
+}
+
+int bhah_hip_ensemble_bind(bhah_ensemble *E, const bhah_MoL_gridfunctions *gfs, const double *dt) {
+  if (!E || !gfs) return fail(BHAH_ERR_ARG, "NULL argument");
+  for (size_t m = 0; m < E->P.size(); m++) {
+    const bhah_patch *P = E->P[m];
+    const bhah_MoL_gridfunctions &g = gfs[m];
+    if (!g.y_n_gfs || !g.y_nplus1_running_total_gfs || !g.k_odd_gfs || !g.k_even_gfs || !g.auxevol_gfs)
+      return fail(BHAH_ERR_ARG, "member %d: MoL gridfunctions not allocated", (int)m);
+    bhah_member &M = E->hmem[m];
+    M.kp = P->kp(dt ? dt[m] : P->dt, 0);
+    M.rfm0 = P->d_rfm0;
+    M.rfm1 = P->d_rfm1;
+    M.xx0 = P->d_xx0;
+    M.arr[0] = g.y_n_gfs;
+    M.arr[1] = g.y_nplus1_running_total_gfs;
+    M.arr[2] = g.k_odd_gfs;
+    M.arr[3] = g.k_even_gfs;
+    M.arr[4] = g.auxevol_gfs;
+    M.coord = P->d.coord;
+    M.AMPL = P->d.grid_physical_size;
+    M.SINHW = P->d.SINHW;
+  }
+  HIP_TRY(hipMemcpy(E->d_mem, E->hmem.data(), sizeof(bhah_member) * E->hmem.size(), hipMemcpyHostToDevice));
+  E->bound = true;
+  E->invalidate_graph();
+  return BHAH_OK;
+}
+
+int bhah_hip_ensemble_set_stream(bhah_ensemble *E, void *s) {
+  if (!E) return fail(BHAH_ERR_ARG, "NULL ensemble");
+  E->stream = (hipStream_t)s;
+  E->invalidate_graph();
+  return BHAH_OK;
+}
+
+int bhah_hip_ensemble_select_schedule(bhah_ensemble *E, int kind, int index) {
+  if (!E) return fail(BHAH_ERR_ARG, "NULL ensemble");
+  const int n = kind == BHAH_KERNEL_RHS ? E->k->n_rhs : kind == BHAH_KERNEL_RICCI ? E->k->n_ricci : -1;
+  if (n < 0) return fail(BHAH_ERR_ARG, "unknown kernel kind %d", kind);
+  if (index < 0 || index >= n) return fail(BHAH_ERR_ARG, "schedule index %d out of range [0,%d)", index, n);
+  const BssnCandidate &c = (kind == BHAH_KERNEL_RHS ? E->k->rhs : E->k->ricci)[index];
+  if (!c.ens) return fail(BHAH_ERR_UNSUPPORTED, "schedule %s has no batched form", c.name);
+  if (kind == BHAH_KERNEL_RHS) E->rhs_choice = index;
+  else E->ricci_choice = index;
+  E->invalidate_graph();
+  return BHAH_OK;
+}
+
+int bhah_hip_ensemble_set_launch(bhah_ensemble *E, int tile_x, int tile_y, int max_workgroups) {
+  if (!E) return fail(BHAH_ERR_ARG, "NULL ensemble");
+  if (tile_x * tile_y != 256 || tile_x < 16 || tile_x % 16) return fail(BHAH_ERR_ARG, "tile must be 256 threads, tile_x a multiple of 16");
+  if (max_workgroups < 0) return fail(BHAH_ERR_ARG, "max_workgroups must be >= 0 (0: one per tile)");
+  E->tile_x = tile_x;
+  E->tile_y = tile_y;
+  E->max_wg = max_workgroups;
+  E->invalidate_graph();
+  return ensemble_build_items(E);
+}
+
+int bhah_hip_ensemble_step_n(bhah_ensemble *E, int nsteps, int use_graph) {
+  if (!E || nsteps < 0) return fail(BHAH_ERR_ARG, "bad arguments");
+  if (!E->bound) return fail(BHAH_ERR_ARG, "ensemble arrays not bound (bhah_hip_ensemble_bind)");
+  if (!use_graph) {
+    for (int n = 0; n < nsteps; n++) {
+      int rc = ensemble_enqueue_step(E);
+      if (rc) return rc;
+    }
+    return BHAH_OK;
+  }
+  if (!E->graph_exec) {
+    hipStream_t cap = E->stream;
+    bool tmp = false;
+    if (!cap) {
+      HIP_TRY(hipStreamCreate(&cap));
+      tmp = true;
+    }
+    hipStream_t saved = E->stream;
+    E->stream = cap;
+    HIP_TRY(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+    int rc = ensemble_enqueue_step(E);
+    hipGraph_t graph;
+    hipError_t e = hipStreamEndCapture(cap, &graph);
+    E->stream = saved;
+    if (tmp) hipStreamDestroy(cap);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(BHAH_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&E->graph_exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(BHAH_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
+  }
+  for (int n = 0; n < nsteps; n++) HIP_TRY(hipGraphLaunch(E->graph_exec, E->stream));
+  return BHAH_OK;
+}
+
+int bhah_hip_ensemble_synchronize(bhah_ensemble *E) {
+  if (!E) return fail(BHAH_ERR_ARG, "NULL ensemble");
+  HIP_TRY(hipStreamSynchronize(E->stream));
   return BHAH_OK;
 }
 

@@ -116,6 +116,16 @@ SIGNATURES: Dict[str, tuple] = {
     "bhah_hip_wave_rhs_eval": (_I, [_P, _P, _P]),
     "bhah_hip_wave_apply_bcs": (_I, [_P, _P]),
     "bhah_hip_diagnostics_constraints": (_I, [_P, _P, _P, _P, ctypes.POINTER(_D)]),
+    "bhah_hip_ensemble_create": (_I, [ctypes.POINTER(PatchDesc), _I, ctypes.POINTER(_P)]),
+    "bhah_hip_ensemble_destroy": (_I, [_P]),
+    "bhah_hip_ensemble_size": (_I, [_P]),
+    "bhah_hip_ensemble_member": (_I, [_P, _I, ctypes.POINTER(_P)]),
+    "bhah_hip_ensemble_bind": (_I, [_P, ctypes.POINTER(MoLGridfunctions), ctypes.POINTER(_D)]),
+    "bhah_hip_ensemble_set_stream": (_I, [_P, _P]),
+    "bhah_hip_ensemble_select_schedule": (_I, [_P, _I, _I]),
+    "bhah_hip_ensemble_set_launch": (_I, [_P, _I, _I, _I]),
+    "bhah_hip_ensemble_step_n": (_I, [_P, _I, _I]),
+    "bhah_hip_ensemble_synchronize": (_I, [_P]),
 }
 
 _LIB: Optional[ctypes.CDLL] = None
@@ -200,15 +210,18 @@ def _ptr(t) -> int:
 class Patch:
     """One grid patch: geometry, boundary structs and kernels on the current device."""
 
-    def __init__(self, cfg: PatchConfig, stream=None, tuned: bool = True):
+    def __init__(self, cfg: PatchConfig, stream=None, tuned: bool = True, handle=None):
         import torch
 
         self.lib = load_library()
         self.cfg = cfg
         self._desc = cfg.to_c()
-        h = ctypes.c_void_p()
-        _check(self.lib.bhah_hip_patch_create(ctypes.byref(self._desc), ctypes.byref(h)))
-        self.handle = h
+        self.owned = handle is None
+        if handle is None:
+            h = ctypes.c_void_p()
+            _check(self.lib.bhah_hip_patch_create(ctypes.byref(self._desc), ctypes.byref(h)))
+            handle = h
+        self.handle = handle
         info = PatchInfo()
         _check(self.lib.bhah_hip_patch_get_info(self.handle, ctypes.byref(info)))
         self.info = info
@@ -230,7 +243,8 @@ class Patch:
 
     def close(self) -> None:
         if getattr(self, "handle", None):
-            self.lib.bhah_hip_patch_destroy(self.handle)
+            if self.owned:
+                self.lib.bhah_hip_patch_destroy(self.handle)
             self.handle = None
 
     def __del__(self):
@@ -356,3 +370,75 @@ class MoLState:
         _check(self.patch.lib.bhah_hip_MoL_step_n(self.patch.handle, ctypes.byref(self.c), dt, n, int(use_graph)))
         self.nn += n
         self.time = self.nn * dt
+
+
+class Ensemble:
+    """Many independent BSSN evolutions of one kernel build advanced together.
+
+    Each member owns torch-allocated MoL arrays (``states[m]``, a ``MoLState``
+    over the member's patch); ``steps(n)`` advances every member n RK4 steps
+    with its own timestep through the batched kernels (one persistent launch per
+    stage for the whole ensemble, a hipGraph per step).
+    """
+
+    def __init__(self, cfgs, stream=None, dts=None):
+        import torch
+
+        self.lib = load_library()
+        self.cfgs = list(cfgs)
+        descs = (PatchDesc * len(self.cfgs))(*[c.to_c() for c in self.cfgs])
+        h = ctypes.c_void_p()
+        _check(self.lib.bhah_hip_ensemble_create(descs, len(self.cfgs), ctypes.byref(h)))
+        self.handle = h
+        s = stream if stream is not None else torch.cuda.current_stream()
+        _check(self.lib.bhah_hip_ensemble_set_stream(self.handle, ctypes.c_void_p(s.cuda_stream)))
+        self.members = []
+        for m, c in enumerate(self.cfgs):
+            ph = ctypes.c_void_p()
+            _check(self.lib.bhah_hip_ensemble_member(self.handle, m, ctypes.byref(ph)))
+            self.members.append(Patch(c, stream=s, tuned=False, handle=ph))
+        self.states = [MoLState(P) for P in self.members]
+        self.dts = [P.dt for P in self.members] if dts is None else list(dts)
+        self.choice = {KERNEL_RHS: 0, KERNEL_RICCI: 0}
+        self.tile, self.max_wg = (64, 4), 0
+        self.bind()
+
+    def bind(self):
+        arr = (MoLGridfunctions * len(self.states))(*[st.c for st in self.states])
+        dts = (ctypes.c_double * len(self.dts))(*self.dts)
+        _check(self.lib.bhah_hip_ensemble_bind(self.handle, arr, dts))
+
+    def schedules(self, kind: int):
+        return self.members[0].schedules(kind)
+
+    def select_schedule(self, kind: int, name_or_index):
+        names = self.schedules(kind)
+        idx = names.index(name_or_index) if isinstance(name_or_index, str) else int(name_or_index)
+        _check(self.lib.bhah_hip_ensemble_select_schedule(self.handle, kind, idx))
+        self.choice[kind] = idx
+
+    def set_launch(self, tile_x: int, tile_y: int, max_workgroups: int = 0):
+        _check(self.lib.bhah_hip_ensemble_set_launch(self.handle, tile_x, tile_y, max_workgroups))
+        self.tile, self.max_wg = (tile_x, tile_y), max_workgroups
+
+    def steps(self, n: int, use_graph: bool = True):
+        _check(self.lib.bhah_hip_ensemble_step_n(self.handle, n, int(use_graph)))
+        for st, dt in zip(self.states, self.dts):
+            st.nn += n
+            st.time = st.nn * dt
+
+    def synchronize(self):
+        _check(self.lib.bhah_hip_ensemble_synchronize(self.handle))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            for P in self.members:
+                P.handle = None
+            self.lib.bhah_hip_ensemble_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -17,6 +17,7 @@ inline hipError_t hipGetLastError() { return hipSuccess; }
 
 #define __global__
 #define __device__
+#define __forceinline__ inline
 #define __launch_bounds__(...)
 #define __builtin_amdgcn_sched_barrier(x) ((void)0)
 

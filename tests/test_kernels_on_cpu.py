@@ -110,3 +110,57 @@ def test_spherical_fd4_schedules_on_cpu(oracle_lib):
 @pytest.mark.parametrize("tag", list(CASES))
 def test_every_schedule_on_cpu(oracle_lib, tag):
     _check(oracle_lib, tag, schedules(tag), [(64, 4), (32, 8)])
+
+
+ENS_DRIVER = os.path.join(HERE, "kernel_cpu", "ens_driver.cpp")
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
+@pytest.mark.parametrize("kind,cand", [("ricci", "r2_lb1_sl_pf16"), ("rhs", "p3_lb1_sl_ln_pf16")])
+def test_ensemble_kernels_on_cpu(oracle_lib, kind, cand):
+    """Batched-ensemble form of a schedule over members of different grids, with a
+    persistent grid smaller than the tile list, matches the oracle per member."""
+    tag = "Spherical_sym2_KO0_fd4"
+    grids = [(12, 8, 2), (16, 6, 4), (10, 4, 2)]
+    Os, ys, auxs, refs = [], [], [], []
+    for k, N in enumerate(grids):
+        O = oracle_lib.OracleGrid(1, N, 4, 5.0 + k, 0.2, variant=0)
+        y = ID.brill_lindquist(O.coord, O.xx, O.g.p0, O.g.p1)
+        y = np.ascontiguousarray(ID.smooth_perturbation(y, O.xx, 10 + k))
+        O.apply_bcs_outerextrap_and_inner(y)
+        aux = O.ricci(y)
+        Os.append(O), ys.append(y), auxs.append(aux)
+        refs.append(aux if kind == "ricci" else O.rhs(y, aux, O.params()))
+    wd = tempfile.mkdtemp(prefix="kens_")
+    try:
+        src = os.path.join(GEN, f"bssn_{tag}_{kind}_{cand}.hip")
+        exe = os.path.join(wd, "ens")
+        subprocess.run(["g++", "-std=c++17", "-O1", "-g", "-fsanitize=address", "-fno-omit-frame-pointer", "-w",
+                        f"-I{SHIM}", f'-DKERNEL_FILE="{src}"', f"-DLAUNCH=launch_bssn_ens_{kind}_{tag}_{cand}",
+                        f"-DKIND_RHS={int(kind == 'rhs')}", ENS_DRIVER, "-o", exe, "-lm"],
+                       check=True, capture_output=True)
+        inp, outp = os.path.join(wd, "in.bin"), os.path.join(wd, "out.bin")
+        with open(inp, "wb") as f:
+            np.array([len(grids), 64, 4, 3], dtype=np.int32).tofile(f)
+            for O, y, aux in zip(Os, ys, auxs):
+                np.array([*O.Nx, O.NG], dtype=np.int32).tofile(f)
+                np.array([1.0 / (O.xx[d][1] - O.xx[d][0]) for d in range(3)] + [2.0, 0.3, 0.3, 1.0]).tofile(f)
+                y.tofile(f)
+                if kind == "rhs":
+                    aux.tofile(f)
+                O.rfm0.tofile(f)
+                O.rfm1.tofile(f)
+        r = subprocess.run([exe, inp, outp], capture_output=True, text=True,
+                           env=dict(os.environ, ASAN_OPTIONS="detect_leaks=0"))
+        assert r.returncode == 0, r.stderr[-3000:]
+        flat = np.fromfile(outp)
+        nout = 24 if kind == "rhs" else 6
+        off = 0
+        for O, ref in zip(Os, refs):
+            n = nout * O.Ntot
+            out = flat[off:off + n].reshape(nout, *O.Nx[::-1])
+            off += n
+            er = rel_err(interior(out, O.NG), interior(ref, O.NG))
+            assert er.max() < 1e-12, er
+    finally:
+        shutil.rmtree(wd, ignore_errors=True)
