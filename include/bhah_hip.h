@@ -183,6 +183,16 @@ int bhah_hip_wave_apply_bcs(bhah_patch *patch, double *gfs);
 int bhah_hip_diagnostics_constraints(bhah_patch *patch, const double *auxevol_gfs, const double *in_gfs, double *diag,
                                      double *l2_H);
 
+/* Reverse mode of MoL_step_forward_in_time (one RK4 step; no reference counterpart --
+ * the reference has no adjoint).  Given y_n, the state at the start of the step, and
+ * lam, the adjoint (gradient of a scalar loss) with respect to the state after the
+ * step, overwrite lam with the adjoint with respect to y_n.  The step's stages are
+ * recomputed from y_n into a patch-owned workspace (allocated on the first call).
+ * Builds with adjoint kernels: Spherical, symmetry_axes 4, no KO, fd_order 4, radiation
+ * outer BCs.  Derivatives are those of the discrete step; the upwind stencil choice is
+ * piecewise constant in beta^i and contributes none. */
+int bhah_hip_MoL_step_adjoint(bhah_patch *patch, const double *y_n, double dt, double *lam);
+
 /* Batched ensembles: many independent BSSN patches of one kernel build, with uneven
  * grids, advanced together.  No single reference counterpart: it batches
  * MoL_step_forward_in_time (MoLtimestepping/MoL_step_forward_in_time.py:39-507) over

@@ -992,6 +992,8 @@ def generate(outdir: str) -> List[str]:
     hdr = "// Generated by nrpy_amd/codegen/hip_gen.py -- do not edit.\n#include <hip/hip_runtime.h>\n#include <math.h>\n" + KPARAMS
     meta = {"flops_per_point": {}, "candidates": {}}
     table_decl, table_rows = [], []
+    from .adjoint_gen import ADJOINT_BUILDS as adjoint_builds
+
     for v, order, cands, ens in BSSN_BUILDS:
         tag = kernel_name_tag(v, order)
         path = os.path.join(outdir, f"bssn_{tag}.hip")
@@ -1021,8 +1023,29 @@ def generate(outdir: str) -> List[str]:
             f"    {{{COORD_ENUM[v.coord]}, {sym_mask}, {int(v.enable_KO)}, {order}, \"{tag}\", "
             f"{arrays['rhs']}, {len(cands['rhs'])}, {arrays['ricci']}, {len(cands['ricci'])}, "
             f"launch_bssn_rk_det_{tag}, launch_bssn_detgamma_{tag}, launch_bssn_constraints_{tag}, "
-            f"{f'launch_bssn_ens_rk_det_{tag}' if ens else 'nullptr'}}},"
+            f"{f'launch_bssn_ens_rk_det_{tag}' if ens else 'nullptr'}, "
+            + (f"adj_nslots_{tag}, launch_adj_rhs_{tag}, launch_adj_ricci_{tag}, launch_adj_det_{tag}, "
+               f"launch_adj_gather_{tag}}},"
+               if (v, order) in adjoint_builds else "nullptr, nullptr, nullptr, nullptr, nullptr},")
         )
+        if (v, order) in adjoint_builds:
+            from . import adjoint_gen
+
+            path = os.path.join(outdir, f"adjoint_{tag}.hip")
+            with open(path, "w", encoding="utf-8") as f:
+                f.write(hdr + adjoint_gen.adjoint_tu(v, order))
+            paths.append(path)
+            table_decl += [
+                f'extern "C" int adj_nslots_{tag}(void);',
+                f'extern "C" hipError_t launch_adj_rhs_{tag}(dim3, dim3, hipStream_t, const double *, const double *, '
+                f"const double *, double *, const double *, const double *, bhah_kparams);",
+                f'extern "C" hipError_t launch_adj_ricci_{tag}(dim3, dim3, hipStream_t, const double *, double *, '
+                f"const double *, const double *, bhah_kparams);",
+                f'extern "C" hipError_t launch_adj_det_{tag}(dim3, dim3, hipStream_t, const double *, double *, '
+                f"const double *, const double *, bhah_kparams);",
+                f'extern "C" hipError_t launch_adj_gather_{tag}(dim3, dim3, hipStream_t, const double *, double *, '
+                f"bhah_kparams);",
+            ]
         if ens:
             table_decl.append(
                 f'extern "C" hipError_t launch_bssn_ens_rk_det_{tag}(dim3, dim3, hipStream_t, const bhah_member *, '

@@ -43,6 +43,15 @@ struct BssnKernels {
   hipError_t (*constraints)(dim3, dim3, hipStream_t, const double *, const double *, double *, const double *,
                             const double *, bhah_kparams);
   hipError_t (*ens_rk_det)(dim3, dim3, hipStream_t, const bhah_member *, const bhah_item *, int, int, int, int);
+  // reverse-mode kernels of one RK4 step (builds with adjoints only)
+  int (*adj_nslots)(void);
+  hipError_t (*adj_rhs)(dim3, dim3, hipStream_t, const double *, const double *, const double *, double *,
+                        const double *, const double *, bhah_kparams);
+  hipError_t (*adj_ricci)(dim3, dim3, hipStream_t, const double *, double *, const double *, const double *,
+                          bhah_kparams);
+  hipError_t (*adj_det)(dim3, dim3, hipStream_t, const double *, double *, const double *, const double *,
+                        bhah_kparams);
+  hipError_t (*adj_gather)(dim3, dim3, hipStream_t, const double *, double *, bhah_kparams);
 };
 #include "gen/kernel_table.inc"
 
@@ -102,6 +111,8 @@ struct bhah_patch {
   // selected schedules (indices into k->rhs / k->ricci) and interior tile shape
   int rhs_choice = 0, ricci_choice = 0;
   int tile_x = 64, tile_y = 4;
+  // adjoint workspace (allocated on the first MoL_step_adjoint call)
+  double *d_adj = nullptr;
   // cached graph for MoL_step_n
   hipGraphExec_t graph_exec = nullptr;
   bhah_MoL_gridfunctions graph_gf{};
@@ -284,6 +295,91 @@ bc_inner_kernel(const int *__restrict__ dst, const int *__restrict__ src, const 
   double *f = &gfs[gf * Ntot];
   const double v = f[src[p]];
   f[dst[p]] = ((mask[p] >> gf_parity[gf]) & 1u) ? -v : v;
+}
+
+// ---- Adjoints of the boundary conditions (reverse mode; linear maps).
+
+// Weights and flat indices of the radial derivative at one point, as partial_r_f forms it.
+__device__ __forceinline__ int partial_r_taps(const RadStencil &st, int R, int Nx0, long idx, int i0, int FACE0,
+                                             long *at, double *w) {
+  int off = FACE0;
+  if (i0 < R)
+    off = R - i0;
+  else if (i0 > Nx0 - R - 1)
+    off = (Nx0 - R - 1) - i0;
+  const int k = off + R;
+  for (int j = 0; j < st.n[k]; j++) {
+    at[j] = idx + st.off[k][j];
+    w[j] = st.w[k][j];
+  }
+  return st.n[k];
+}
+
+// Reverse of one radiation-BC layer: the forward set rhs[p] from rhs[p_int] and the
+// stage input f around p and p_int; push rhs[p]'s adjoint to rhs[p_int] (distinct per
+// point of a layer) and, atomically, to f, then clear it.
+__global__ void __launch_bounds__(256)
+bc_radiation_adjoint_kernel(const int16_t *__restrict__ pts, int npts, int ngfs, double *__restrict__ lam_rhs,
+                            double *__restrict__ lam_f, const double *__restrict__ xx0,
+                            const double *__restrict__ wavespeed, RadStencil st, int R, int coord, double AMPL,
+                            double SINHW, int Nx0, long s1, long s2, long Ntot, double invdxx0) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)npts * ngfs) return;
+  const int p = (int)(t % npts), gf = (int)(t / npts);
+  const int16_t *q = &pts[6 * p];
+  const int i0 = q[0], i1 = q[1], i2 = q[2];
+  const int j0 = i0 + q[3], j1 = i1 + q[4], j2 = i2 + q[5];
+  const long idx = i0 + s1 * i1 + s2 * i2;
+  const long idx_int = j0 + s1 * j1 + s2 * j2;
+  double *lr = &lam_rhs[gf * Ntot];
+  double *lf = &lam_f[gf * Ntot];
+  const double lam = lr[idx];
+  lr[idx] = 0.0;
+  if (lam == 0.0) return;
+  double r, dxr, r_int, dxr_int;
+  r_and_dx0dr(coord, AMPL, SINHW, xx0[i0], r, dxr);
+  r_and_dx0dr(coord, AMPL, SINHW, xx0[j0], r_int, dxr_int);
+  const double c = wavespeed[gf];
+  const double ratio = (r_int * r_int * r_int) / (r * r * r);
+  lr[idx_int] += ratio * lam;
+  long at[8];
+  double w[8];
+  int n = partial_r_taps(st, R, Nx0, idx, i0, q[3], at, w);
+  for (int j = 0; j < n; j++) atomicAdd(&lf[at[j]], -c * dxr * invdxx0 * w[j] * lam);
+  atomicAdd(&lf[idx], -c / r * lam);
+  n = partial_r_taps(st, R, Nx0, idx_int, j0, q[3], at, w);
+  for (int j = 0; j < n; j++) atomicAdd(&lf[at[j]], ratio * c * dxr_int * invdxx0 * w[j] * lam);
+  atomicAdd(&lf[idx_int], ratio * c / r_int * lam);
+}
+
+// Reverse of gf[dst] = +-gf[src]: lam[src] += +-lam[dst], lam[dst] = 0.
+__global__ void __launch_bounds__(256)
+bc_inner_adjoint_kernel(const int *__restrict__ dst, const int *__restrict__ src,
+                        const unsigned short *__restrict__ mask, int npts, int ngfs,
+                        const int *__restrict__ gf_parity, double *__restrict__ lam, long Ntot) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)npts * ngfs) return;
+  const int p = (int)(t % npts), gf = (int)(t / npts);
+  double *l = &lam[gf * Ntot];
+  const double v = l[dst[p]];
+  l[dst[p]] = 0.0;
+  if (v != 0.0) atomicAdd(&l[src[p]], ((mask[p] >> gf_parity[gf]) & 1u) ? -v : v);
+}
+
+// out = y + c0 k0 + c1 k1 + c2 k2 + c3 k3 over n values (RK stage arguments).
+__global__ void __launch_bounds__(256)
+rk_combine_kernel(double *__restrict__ out, const double *__restrict__ y, const double *__restrict__ k, long n,
+                  double c0, double c1, double c2, double c3) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  out[t] = y[t] + c0 * k[t] + c1 * k[n + t] + c2 * k[2 * n + t] + c3 * k[3 * n + t];
+}
+
+// dst += a * src
+__global__ void __launch_bounds__(256)
+axpy_kernel(double *__restrict__ dst, const double *__restrict__ src, double a, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) dst[t] += a * src[t];
 }
 
 // Volume-weighted sums over the interior for the constraint norms.
@@ -664,6 +760,7 @@ int bhah_hip_patch_destroy(bhah_patch *P) {
   hipFree(P->d_finf);
   hipFree(P->d_red);
   hipFree(P->d_gf_parity);
+  hipFree(P->d_adj);
   for (int gz = 0; gz < 8; gz++)
     for (int d = 0; d < 3; d++) hipFree(P->d_outer[gz][d]);
   if (P->own_stream) hipStreamDestroy(P->stream);
@@ -1006,6 +1103,109 @@ int bhah_hip_diagnostics_constraints(bhah_patch *P, const double *aux, const dou
     l2_H[1] = sqrt(red[1]);           // sqrt of the volume integral of H^2
     l2_H[2] = red[0];                 // proper volume
   }
+  return BHAH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Reverse mode of one RK4 step (BSSN, radiation outer BCs).  The step's stages are
+// recomputed from y_n into a patch-owned workspace, then the adjoint runs backwards:
+// det-enforcement Jacobians, RK combinations, boundary conditions, rhs_eval and
+// Ricci_eval transposes and the transposed stencils.
+
+static int adjoint_rhs_bcs(bhah_patch *P, double *lam_k, double *lam_f) {
+  // forward order: radiation layers gz = 0..NG-1 (directions 0..2), then inner parity
+  const int n_in = (int)P->inner_dst.size();
+  if (n_in) {
+    dim3 g, b;
+    launch_1d((long)n_in * P->ngfs, g, b);
+    hipLaunchKernelGGL(bc_inner_adjoint_kernel, g, b, 0, P->stream, P->d_inner_dst, P->d_inner_src, P->d_inner_mask,
+                       n_in, P->ngfs, P->d_gf_parity, lam_k, P->Ntot);
+    HIP_TRY(hipGetLastError());
+  }
+  const int R = (P->d.radiation_bc_fd_order > 0 ? P->d.radiation_bc_fd_order : 4) / 2;
+  RadStencil st;
+  rad_stencil(R, st);
+  for (int gz = P->NG - 1; gz >= 0; gz--)
+    for (int d = 2; d >= 0; d--) {
+      const int n = (int)(P->outer[gz][d].size() / 6);
+      if (!n) continue;
+      dim3 g, b;
+      launch_1d((long)n * P->ngfs, g, b);
+      hipLaunchKernelGGL(bc_radiation_adjoint_kernel, g, b, 0, P->stream, P->d_outer[gz][d], n, P->ngfs, lam_k, lam_f,
+                         P->d_xx0, P->d_wavespeed, st, R, P->d.coord, P->d.grid_physical_size, P->d.SINHW, P->Nx[0],
+                         (long)P->Nx[0], (long)P->Nx[0] * P->Nx[1], P->Ntot, P->invdxx[0]);
+      HIP_TRY(hipGetLastError());
+    }
+  return BHAH_OK;
+}
+
+extern "C" int bhah_hip_MoL_step_adjoint(bhah_patch *P, const double *y_n, double dt, double *lam) {
+  if (!P || !P->k || !y_n || !lam) return fail(BHAH_ERR_ARG, "bad arguments");
+  if (!P->k->adj_rhs) return fail(BHAH_ERR_UNSUPPORTED, "kernel build %s has no adjoint kernels", P->k->tag);
+  if (P->d.outer_bc != BHAH_OUTER_BC_RADIATION) return fail(BHAH_ERR_UNSUPPORTED, "adjoint needs radiation outer BCs");
+  const long N = P->Ntot, nv = N * P->ngfs;
+  const int nslots = P->k->adj_nslots();
+  // workspace: Y2..Y4 (3 nv), K1..K4 (4 nv), AUX1..4 (4 x 6N), y_run, u, lamK1..4 (4 nv), lamY, lamYN, S
+  const size_t total = (size_t)(3 + 4 + 1 + 1 + 4 + 2) * nv + (size_t)4 * 6 * N + (size_t)nslots * N;
+  if (!P->d_adj) HIP_TRY(hipMalloc((void **)&P->d_adj, total * sizeof(double)));
+  double *w = P->d_adj;
+  double *Y[4] = {const_cast<double *>(y_n), w, w + nv, w + 2 * nv};
+  double *K = w + 3 * nv;  // 4 consecutive arrays
+  double *yrun = w + 7 * nv, *u = w + 8 * nv, *lamK = w + 9 * nv, *lamY = w + 13 * nv, *lamYN = w + 14 * nv;
+  double *AUX = w + 15 * nv;
+  double *S = AUX + 24 * N;
+  hipStream_t st = P->stream;
+  dim3 ga, ba, gi, bi, g1, b1;
+  P->all_dims(ga, ba);
+  P->interior_dims(gi, bi);
+  launch_1d(nv, g1, b1);
+  int rc;
+  // ---- forward recompute of the step's stages
+  for (int s = 1; s <= 4; s++) {
+    double *in = Y[s - 1], *k = K + (s - 1) * nv, *aux = AUX + (s - 1) * 6 * N;
+    if ((rc = bhah_hip_Ricci_eval(P, in, aux))) return rc;
+    if ((rc = bhah_hip_rhs_eval(P, aux, in, k))) return rc;
+    if ((rc = bhah_hip_apply_bcs_outerradiation_and_inner(P, in, k))) return rc;
+    if (s < 4) HIP_TRY(P->k->rk_det(ga, ba, st, k, y_n, yrun, Y[s], P->d_rfm0, P->d_rfm1, P->kp(dt, s)));
+  }
+  // ---- reverse
+  const double a_stage[4] = {0.5 * dt, 0.5 * dt, dt, 0.0};  // Y[s] = D(y_n + a_stage[s-1] K_s), s = 1..3
+  const double b_final[4] = {dt / 6.0, dt / 3.0, dt / 3.0, dt / 6.0};
+  HIP_TRY(hipMemsetAsync(lamK, 0, 4 * nv * sizeof(double), st));
+  // y_{n+1} = D(y_n + sum b K): lam_u = J_D^T lam
+  hipLaunchKernelGGL(rk_combine_kernel, g1, b1, 0, st, u, y_n, K, nv, b_final[0], b_final[1], b_final[2], b_final[3]);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(lamYN, lam, nv * sizeof(double), hipMemcpyDeviceToDevice, st));
+  HIP_TRY(P->k->adj_det(ga, ba, st, u, lamYN, P->d_rfm0, P->d_rfm1, P->kp(dt, 0)));
+  for (int j = 0; j < 4; j++) {
+    hipLaunchKernelGGL(axpy_kernel, g1, b1, 0, st, lamK + j * nv, lamYN, b_final[j], nv);
+    HIP_TRY(hipGetLastError());
+  }
+  for (int s = 4; s >= 1; s--) {
+    double *in = Y[s - 1], *aux = AUX + (s - 1) * 6 * N, *lk = lamK + (s - 1) * nv;
+    HIP_TRY(hipMemsetAsync(lamY, 0, nv * sizeof(double), st));
+    if ((rc = adjoint_rhs_bcs(P, lk, lamY))) return rc;
+    HIP_TRY(hipMemsetAsync(S, 0, (size_t)nslots * N * sizeof(double), st));
+    HIP_TRY(P->k->adj_rhs(gi, bi, st, in, aux, lk, S, P->d_rfm0, P->d_rfm1, P->kp(dt, 0)));
+    HIP_TRY(P->k->adj_ricci(gi, bi, st, in, S, P->d_rfm0, P->d_rfm1, P->kp(dt, 0)));
+    HIP_TRY(P->k->adj_gather(g1, b1, st, S, lamY, P->kp(dt, 0)));
+    if (s == 1) {
+      hipLaunchKernelGGL(axpy_kernel, g1, b1, 0, st, lamYN, lamY, 1.0, nv);
+      HIP_TRY(hipGetLastError());
+      break;
+    }
+    // Y[s-1] = D(u), u = y_n + a K_{s-1}
+    double c[4] = {0, 0, 0, 0};
+    c[s - 2] = a_stage[s - 2];
+    hipLaunchKernelGGL(rk_combine_kernel, g1, b1, 0, st, u, y_n, K, nv, c[0], c[1], c[2], c[3]);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(P->k->adj_det(ga, ba, st, u, lamY, P->d_rfm0, P->d_rfm1, P->kp(dt, 0)));
+    hipLaunchKernelGGL(axpy_kernel, g1, b1, 0, st, lamYN, lamY, 1.0, nv);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(axpy_kernel, g1, b1, 0, st, lamK + (s - 2) * nv, lamY, a_stage[s - 2], nv);
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipMemcpyAsync(lam, lamYN, nv * sizeof(double), hipMemcpyDeviceToDevice, st));
   return BHAH_OK;
 }
 

@@ -116,6 +116,7 @@ SIGNATURES: Dict[str, tuple] = {
     "bhah_hip_wave_rhs_eval": (_I, [_P, _P, _P]),
     "bhah_hip_wave_apply_bcs": (_I, [_P, _P]),
     "bhah_hip_diagnostics_constraints": (_I, [_P, _P, _P, _P, ctypes.POINTER(_D)]),
+    "bhah_hip_MoL_step_adjoint": (_I, [_P, _P, _D, _P]),
     "bhah_hip_ensemble_create": (_I, [ctypes.POINTER(PatchDesc), _I, ctypes.POINTER(_P)]),
     "bhah_hip_ensemble_destroy": (_I, [_P]),
     "bhah_hip_ensemble_size": (_I, [_P]),
@@ -339,6 +340,10 @@ class Patch:
 
     def synchronize(self):
         _check(self.lib.bhah_hip_synchronize(self.handle))
+
+    def MoL_step_adjoint(self, y_n, dt: float, lam):
+        """lam (adjoint of the state after one RK4 step from y_n) <- adjoint of y_n."""
+        _check(self.lib.bhah_hip_MoL_step_adjoint(self.handle, _ptr(y_n), dt, _ptr(lam)))
 
 
 class MoLState:

@@ -11,6 +11,7 @@ struct dim3 {
 };
 typedef int hipError_t;
 typedef void *hipStream_t;
+#define __constant__
 static const hipError_t hipSuccess = 0;
 inline dim3 blockIdx, threadIdx, blockDim, gridDim;
 inline hipError_t hipGetLastError() { return hipSuccess; }

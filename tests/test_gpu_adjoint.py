@@ -1,0 +1,65 @@
+"""Reverse mode on the GPU: the adjoint of one RK4 step against central finite
+differences of the forward step (dot-product test), and the gradient of the
+lapse-extraction loss (96x16x4, 400 steps, window 50, checkpoints every 20)
+against central finite differences with h = 1e-6."""
+
+import numpy as np
+import pytest
+
+from nrpy_amd import initial_data as ID
+
+pytestmark = pytest.mark.gpu
+
+
+def test_step_adjoint_dot_product(hip):
+    import torch
+    from nrpy_amd.runtime import MoLState, Patch, PatchConfig
+
+    P = Patch(PatchConfig(Nxx=(24, 8, 4), grid_physical_size=10.0, eta=2.0), tuned=False)
+    xx = P.xx()
+    y = ID.brill_lindquist(1, xx, 10.0, 0.2, (0.5, 0.4), (0, 0, 0.5), (0, 0, -0.6))
+    y = ID.smooth_perturbation(y, xx, 3, amplitude=2e-2)
+    y[ID.EVOL_INDEX["vetU0"]] += 0.05  # keep beta^r away from the upwind switch
+    y[ID.EVOL_INDEX["vetU1"]] += 0.05
+    yd = torch.from_numpy(np.ascontiguousarray(y)).cuda()
+    P.apply_bcs_outerextrap_and_inner(yd)
+    rng = np.random.default_rng(11)
+    v = torch.from_numpy(rng.standard_normal(y.shape)).cuda()
+    lam = torch.from_numpy(rng.standard_normal(y.shape)).cuda()
+    h = 1e-6
+
+    def step(z):
+        S = MoLState(P, z.clone())
+        S.step(P.dt)
+        return S.y_n
+
+    fd = (step(yd + h * v) - step(yd - h * v)) / (2 * h)
+    lhs = float((lam * fd).sum())
+    la = lam.clone()
+    P.MoL_step_adjoint(yd, P.dt, la)
+    rhs = float((la * v).sum())
+    print("step adjoint dot product", lhs, rhs)
+    assert abs(lhs - rhs) < 1e-6 * abs(lhs), (lhs, rhs)
+
+
+def test_lapse_loss_gradient_matches_central_differences(hip):
+    import torch
+    from nrpy_amd.adjoint import LapseLossProblem
+
+    prob = LapseLossProblem()
+    theta = torch.tensor([0.5, 0.5, 0.5, -0.5], dtype=torch.float64, requires_grad=True)
+    L = prob.loss(theta)
+    L.backward()
+    g = theta.grad.numpy()
+    h = 1e-6
+    fd = []
+    for k in range(4):
+        tp = theta.detach().numpy().copy()
+        tm = tp.copy()
+        tp[k] += h
+        tm[k] -= h
+        fd.append((prob.loss_value(tp) - prob.loss_value(tm)) / (2 * h))
+    fd = np.array(fd)
+    print("loss", float(L), "adjoint", g, "central FD", fd)
+    assert float(L) > 0
+    assert np.all(np.abs(g - fd) <= 1e-6 * np.abs(fd) + 1e-12 * np.abs(fd).max()), (g, fd)
