@@ -12,7 +12,8 @@ with all fields resident in HBM.  Prints one JSON line.
 
 The line also carries "batched": the 64-member ensemble (seeded masses, separations and
 grids) through the batched path against the same members stepped one after another
-with plain per-patch launches.
+with plain per-patch launches, and "adjoint": the lapse-loss gradient (checkpointed
+forward plus reverse pass) against the plain forward run.
     python bench.py --copy-calibration 512   (plain 512 MiB device copy x3, for
                                               scaling the FETCH_SIZE/WRITE_SIZE counters)
 """
@@ -142,6 +143,44 @@ def bench_batched(steps: int, warmup: int, n: int = 64) -> dict:
     }
 
 
+def bench_adjoint() -> dict:
+    """Lapse-extraction loss on Spherical 96x16x4 (400 RK4 steps, loss over the last 50,
+    16 samples on r = 4, checkpoints every 20): time of the plain forward run against
+    the gradient (checkpointed forward + reverse pass with recomputation)."""
+    import torch
+
+    from nrpy_amd.adjoint import LapseLossProblem
+
+    prob = LapseLossProblem()
+    theta0 = [0.5, 0.5, 0.5, -0.5]
+    prob.loss_value(theta0)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    L = prob.loss_value(theta0)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    theta = torch.tensor(theta0, dtype=torch.float64, requires_grad=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loss = prob.loss(theta)
+    loss.backward()
+    torch.cuda.synchronize()
+    t_grad = time.perf_counter() - t0
+    cfg = prob.cfg
+    npts = cfg.Nxx[0] * cfg.Nxx[1] * cfg.Nxx[2]
+    return {
+        "workload": "bssn_lapse_loss_gradient_spherical_96x16x4_400steps_K20",
+        "loss": L,
+        "gradient_m1_m2_z1_z2": [float(x) for x in theta.grad],
+        "forward_s": t_fwd,
+        "gradient_s": t_grad,
+        "gradient_vs_forward_time": t_grad / t_fwd,
+        "plain_value": npts * cfg.nsteps / t_fwd,
+        "adjoint_value": npts * cfg.nsteps / max(t_grad - t_fwd, 1e-9),
+        "unit": "gridpoint-RK4-steps/s",
+    }
+
+
 def cpu_baseline(c: dict, steps: int = 1) -> dict:
     """The oracle (plain-C restatement) on the host cores, one RK4 step of the same grid."""
     import numpy as np
@@ -181,6 +220,7 @@ def main() -> None:
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-ensemble comparison")
     ap.add_argument("--batched-steps", type=int, default=200)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient timing")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB three times, then exit")
     args = ap.parse_args()
@@ -307,6 +347,8 @@ def main() -> None:
     }
     if not args.no_batched:
         result["batched"] = bench_batched(args.batched_steps, 2)
+    if not args.no_adjoint:
+        result["adjoint"] = bench_adjoint()
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(c)
     print(json.dumps(result))

@@ -63,3 +63,50 @@ def test_lapse_loss_gradient_matches_central_differences(hip):
     print("loss", float(L), "adjoint", g, "central FD", fd)
     assert float(L) > 0
     assert np.all(np.abs(g - fd) <= 1e-6 * np.abs(fd) + 1e-12 * np.abs(fd).max()), (g, fd)
+
+
+def test_gradient_matches_torch_transcription(hip):
+    """On a 24x6x2 patch (8 steps, window 4, checkpoints every 3), the adjoint gradient
+    equals torch autograd through the PyTorch transcription of the same step."""
+    import torch
+    from nrpy_amd.adjoint import LapseLossConfig, LapseLossProblem
+    from tests.torch_transcription import TorchBSSN
+
+    cfg = LapseLossConfig(Nxx=(24, 6, 2), nsteps=8, window=4, checkpoint_every=3, n_sample=6)
+    prob = LapseLossProblem(cfg)
+    theta = torch.tensor([0.5, 0.45, 0.5, -0.55], dtype=torch.float64, requires_grad=True)
+    L = prob.loss(theta)
+    L.backward()
+    g = theta.grad.numpy().copy()
+
+    T = TorchBSSN(prob.P, prob.dt)
+    idx, w = prob.s_idx.cpu(), prob.s_w.cpu()
+    from nrpy_amd.initial_data import cartesian_coords
+
+    X, Y, Z = (torch.from_numpy(a) for a in cartesian_coords(1, prob.xx, 0.0, 0.0))
+    ia, ic = 6, 10  # alpha, cf
+
+    def run(th):
+        r1 = torch.sqrt(X * X + Y * Y + (Z - th[2]) ** 2)
+        r2 = torch.sqrt(X * X + Y * Y + (Z - th[3]) ** 2)
+        psi = 1.0 + th[0] / (2 * r1) + th[1] / (2 * r2)
+        rows = [torch.zeros_like(psi)] * 24
+        rows[ia] = rows[ic] = psi ** -2
+        y = torch.stack(rows)
+        rec = []
+        for n in range(1, cfg.nsteps + 1):
+            y = T.step(y)
+            if n >= cfg.nsteps - cfg.window + 1:
+                rec.append((y.reshape(-1)[idx] * w).sum(1))
+        return torch.stack(rec)
+
+    with torch.no_grad():
+        target = run(torch.tensor(cfg.target, dtype=torch.float64))
+    th = theta.detach().clone().requires_grad_(True)
+    s = run(th)
+    Lt = ((s - target) ** 2).mean()
+    Lt.backward()
+    gt = th.grad.numpy()
+    print("loss", float(L), float(Lt), "adjoint", g, "torch autograd", gt)
+    assert abs(float(L) - float(Lt)) < 1e-9 * float(Lt)
+    assert np.all(np.abs(g - gt) <= 1e-7 * np.abs(gt).max()), (g, gt)
