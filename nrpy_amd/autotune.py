@@ -66,10 +66,14 @@ def apply_tuned(patch) -> Optional[Dict]:
     d = load_choice(patch.cfg)
     if not d:
         return None
-    if d["rhs"] in patch.schedules(KERNEL_RHS):
-        patch.select_schedule(KERNEL_RHS, d["rhs"])
-    if d["ricci"] in patch.schedules(KERNEL_RICCI):
-        patch.select_schedule(KERNEL_RICCI, d["ricci"])
+    from .runtime import BHaHError
+
+    for kind, key in ((KERNEL_RHS, "rhs"), (KERNEL_RICCI, "ricci")):
+        if d[key] in patch.schedules(kind):
+            try:
+                patch.select_schedule(kind, d[key])
+            except BHaHError:
+                pass  # schedule not usable at this grid size: keep the default
     patch.set_tile(*d["tile"])
     return d
 

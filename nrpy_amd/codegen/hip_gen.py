@@ -59,10 +59,16 @@ def _dbl(r: sp.Rational) -> str:
 class StencilEmitter:
     """C expressions for derivative symbols, reading ``in`` around ``idx``."""
 
-    def __init__(self, spec: specs.Spec, fd_order: int, array: str = "in"):
+    def __init__(self, spec: specs.Spec, fd_order: int, array: str = "in", u32: bool = False):
         self.spec = spec
         self.fd_order = fd_order
         self.array = array
+        # u32: buffer loads through a resource descriptor of the array, with one 32-bit
+        # per-lane byte offset shared by every load of the thread and the stencil and
+        # gridfunction offsets as uniform (scalar-register) offsets -- no 64-bit
+        # per-lane address arithmetic or address registers.  PAD keeps every uniform
+        # offset non-negative.
+        self.u32 = u32
         self.R = fd.ghost_zones(fd_order)
         self._collect: List[Tuple[str, str]] | None = None
         self._nload = 0
@@ -87,6 +93,15 @@ class StencilEmitter:
             finally:
                 self._collect = coll
         g = self.spec.gfs.index(gf)
+        if self.u32:
+            t = [f"{g} * Ntot + PAD"]
+            if off[0]:
+                t.append(f"{off[0]}")
+            if off[1]:
+                t.append(f"{off[1]} * s1")
+            if off[2]:
+                t.append(f"{off[2]} * s2")
+            return f"BLD(rsrc_{self.array}, {' + '.join(t)})"
         terms = []
         if off[0]:
             terms.append(f"{off[0]}")
@@ -229,14 +244,15 @@ def split_linear(spec: specs.Spec):
     return cores, terms
 
 
-def _emit_defs(spec: specs.Spec, fd_order: int):
+def _emit_defs(spec: specs.Spec, fd_order: int, u32: bool = False):
     centre, aux, derivs, rfm, params = spec.classify()
-    st = StencilEmitter(spec, fd_order)
+    st = StencilEmitter(spec, fd_order, u32=u32)
     defs: Dict[str, str] = {}
     for n in centre:
         defs[n] = st.load(n, (0, 0, 0))
     for n in aux:
-        defs[n] = f"aux[{spec.aux.index(n)} * Ntot + idx]"
+        defs[n] = (f"BLD(rsrc_aux, {spec.aux.index(n)} * Ntot + PAD)" if u32
+                   else f"aux[{spec.aux.index(n)} * Ntot + idx]")
     for d in derivs:
         defs[d.symbol] = st.expr(d)
     for n in rfm:
@@ -249,7 +265,7 @@ def _emit_defs(spec: specs.Spec, fd_order: int):
 def _ordered_body(spec: specs.Spec, fd_order: int, out_lvalues: List[str], rationals: Dict[str, str],
                   stream_linear: bool = True, order: str = "dfs", barriers: bool = True,
                   cse_scope: str = "joint", linear: str = "stream", prefetch: int = 0,
-                  ahead: int = 12) -> str:
+                  ahead: int = 12, u32: bool = False) -> str:
     """Stencils and CSE'd algebra for one pointwise spec.
 
     With ``stream_linear``, upwind and KO derivatives are consumed as soon as
@@ -268,11 +284,11 @@ def _ordered_body(spec: specs.Spec, fd_order: int, out_lvalues: List[str], ratio
     most B early-issued values in flight, so that load latency overlaps the
     arithmetic of earlier regions instead of stalling the single wave per SIMD.
     """
-    defs = _emit_defs(spec, fd_order)
+    defs = _emit_defs(spec, fd_order, u32)
     derivs = {d.symbol: d for d in spec.classify()[2]}
     centre_names = set(spec.classify()[0]) | set(spec.classify()[1])
     pr = emit.DoublePrinter(rationals)
-    st = StencilEmitter(spec, fd_order)
+    st = StencilEmitter(spec, fd_order, u32=u32)
     emitted: Set[str] = set()
     items: List[tuple] = []  # ("C", text) | ("L", text, group) | ("B",)
     flops = [0]
@@ -556,7 +572,18 @@ XCD_REMAP = """  // Workgroups are dispatched round-robin over the 8 XCDs (each 
 """
 
 
-def _prologue_at(interior: bool) -> str:
+U32_LINES = """  const long PAD = (long)p.NG * (1 + s1 + s2);
+  const unsigned vo = (unsigned)((idx - PAD) * 8);
+  const unsigned uix = (unsigned)(idx - PAD);
+  const __amdgpu_buffer_rsrc_t rsrc_in = __builtin_amdgcn_make_buffer_rsrc((void *)in, 0, (int)(unsigned)(24 * Ntot * 8), 0x00020000);
+  const __amdgpu_buffer_rsrc_t rsrc_aux = __builtin_amdgcn_make_buffer_rsrc((void *)aux, 0, (int)(unsigned)(6 * Ntot * 8), 0x00020000);
+"""
+# Raw buffer load of one double: per-lane byte offset vo, uniform element offset so.
+BLD_MACRO = """#define BLD(r, so) __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64((r), vo, (unsigned)((so) * 8), 0))
+"""
+
+
+def _prologue_at(interior: bool, u32: bool = False) -> str:
     """Prologue of a per-tile device function: tile coordinates come in as bx, by, bz."""
     lo = "p.NG" if interior else "0"
     return f"""  const int i0 = {lo} + (int)(bx * blockDim.x + threadIdx.x);
@@ -565,14 +592,14 @@ def _prologue_at(interior: bool) -> str:
   if (i0 >= p.Nx0 - {lo} || i1 >= p.Nx1 - {lo} || i2 >= p.Nx2 - {lo}) return;
   const long s1 = p.s1, s2 = p.s2, Ntot = p.Ntot;
   const long idx = i0 + s1 * i1 + s2 * i2;
-"""
+{U32_LINES if u32 else ""}"""
 
 
 ENS_SIG = ("dim3 g, dim3 b, hipStream_t s, const bhah_member *mem, const bhah_item *items, int nitems, "
            "int in_sel, int aux_sel, int out_sel")
 
 
-def _ensemble_kernel(kname: str, upw: str, body: str, lb: str) -> str:
+def _ensemble_kernel(kname: str, upw: str, body: str, lb: str, u32: bool = False) -> str:
     """Persistent batched form of one interior kernel: a fixed grid of workgroups
     walks the ensemble's tile list; each tile runs the same per-point code on its
     member's arrays (selected by index from the member table)."""
@@ -580,7 +607,7 @@ def _ensemble_kernel(kname: str, upw: str, body: str, lb: str) -> str:
 static __device__ __forceinline__ void {kname}_tile(const double *__restrict__ in, const double *__restrict__ aux,
         double *__restrict__ out, const double *__restrict__ rfm0, const double *__restrict__ rfm1,
         const bhah_kparams p, const int bx, const int by, const int bz) {{
-{_prologue_at(True)}
+{_prologue_at(True, u32)}
 {upw}
 {body}
 }}
@@ -596,8 +623,9 @@ extern "C" __global__ void __launch_bounds__({lb})
 """
 
 
-def _prologue(interior: bool, xcd: bool = False) -> str:
+def _prologue(interior: bool, xcd: bool = False, u32: bool = False) -> str:
     lo = "p.NG" if interior else "0"
+    tail = U32_LINES if u32 else ""
     if xcd:
         return XCD_REMAP + f"""  const int i0 = {lo} + (int)(bx * blockDim.x + threadIdx.x);
   const int i1 = {lo} + (int)(by * blockDim.y + threadIdx.y);
@@ -605,18 +633,18 @@ def _prologue(interior: bool, xcd: bool = False) -> str:
   if (i0 >= p.Nx0 - {lo} || i1 >= p.Nx1 - {lo}) return;
   const long s1 = p.s1, s2 = p.s2, Ntot = p.Ntot;
   const long idx = i0 + s1 * i1 + s2 * i2;
-"""
+""" + tail
     return f"""  const int i0 = {lo} + (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const int i1 = {lo} + (int)(blockIdx.y * blockDim.y + threadIdx.y);
   const int i2 = {lo} + (int)blockIdx.z;
   if (i0 >= p.Nx0 - {lo} || i1 >= p.Nx1 - {lo}) return;
   const long s1 = p.s1, s2 = p.s2, Ntot = p.Ntot;
   const long idx = i0 + s1 * i1 + s2 * i2;
-"""
+""" + tail
 
 
-def _upwind_prologue(spec: specs.Spec, fd_order: int, rationals, tables: bool = True) -> str:
-    st = StencilEmitter(spec, fd_order)
+def _upwind_prologue(spec: specs.Spec, fd_order: int, rationals, tables: bool = True, u32: bool = False) -> str:
+    st = StencilEmitter(spec, fd_order, u32=u32)
     pr = emit.DoublePrinter(rationals)
     lines = []
     used = emit.free_symbol_names(spec.upwind)
@@ -711,14 +739,16 @@ LAUNCH_BOUNDS = {"lb1": "256", "lb2": "256, 2"}
 
 # Candidate schedules per build: (kind, partition, launch bounds).
 FULL_CANDIDATES = {
-    "rhs": ["p1_lb1", "p3_lb1_sl", "p3_lb1_sl_ln", "p3_lb1_sl_ln_pf16", "p3_lb1_sl_ln_pf8", "p3_lb1_sl_ln_pf12",
-            "p3_lb1_sl_ln_pf24", "p3_lb1_sl_ln_pf16_a6", "p3_lb1_sl_ln_pf16_a24", "p3_lb1_ln_pf16",
-            "p3_lb1_sj_ln_pf16", "p2_lb1_sl_ln_pf8", "p4_lb1_sl_ln_pf12"],
-    "ricci": ["r1_lb1", "r3_lb1", "r2_lb1_sl_pf32", "r2_lb1_sl_pf24", "r2_lb1_sl_pf32_a6", "r2_lb1_sl_pf32_a24",
-              "r3_lb1_sl_pf32", "r2_lb1_sj_pf32", "r3_lb1_pf16"],
+    "rhs": ["p1_lb1", "p3_lb1_sl", "p3_lb1_sj_ln_pf16", "p3_lb1_sl_ln_pf16", "p3_lb1_sj_ln_pf12_u32",
+            "p3_lb1_sj_ln_pf16_u32", "p3_lb1_sj_ln_pf24_u32", "p3_lb1_sl_ln_pf16_u32", "p2_lb1_sj_ln_pf16_u32",
+            "p4_lb1_sj_ln_pf16_u32"],
+    "ricci": ["r1_lb1", "r3_lb1", "r2_lb1_sj_pf32", "r2_lb1_sl_pf24", "r2_lb1_sj_pf16_u32", "r2_lb1_sj_pf24_u32",
+              "r1_lb1_sj_pf24_u32"],
 }
-SMALL_CANDIDATES = {"rhs": ["p1_lb1", "p1_lb1_sl", "p3_lb1_sl", "p1_lb1_sl_ln_pf16", "p3_lb1_sl_ln_pf16"],
-                    "ricci": ["r1_lb1", "r1_lb1_sl", "r2_lb1_sl", "r1_lb1_pf16", "r2_lb1_sl_pf16"]}
+SMALL_CANDIDATES = {"rhs": ["p1_lb1", "p1_lb1_sl", "p3_lb1_sl", "p1_lb1_sl_ln_pf16", "p3_lb1_sl_ln_pf16",
+                            "p1_lb1_sl_ln_pf16_u32", "p3_lb1_sl_ln_pf16_u32"],
+                    "ricci": ["r1_lb1", "r1_lb1_sl", "r2_lb1_sl", "r1_lb1_pf16", "r2_lb1_sl_pf16", "r1_lb1_pf16_u32",
+                              "r2_lb1_sl_pf16_u32"]}
 
 CAND_SIG = ("dim3 g, dim3 b, hipStream_t s, const double *in, const double *aux, double *out, "
             "const double *rfm0, const double *rfm1, bhah_kparams p")
@@ -747,6 +777,7 @@ def candidate_tu(v: specs.Variant, fd_order: int, kind: str, cand: str, ensemble
     prefetch = next((int(o[2:]) for o in opts if o.startswith("pf")), 0)
     ahead = next((int(o[1:]) for o in opts if o[0] == "a" and o[1:].isdigit()), 12)
     xcd = "xcd" in opts
+    u32 = "u32" in opts
     rats: Dict[str, str] = {}
     kernels, calls, total = [], [], 0
     ens_kernels, ens_calls = [], []
@@ -764,19 +795,19 @@ def candidate_tu(v: specs.Variant, fd_order: int, kind: str, cand: str, ensemble
         if kind == "rhs":
             lv = [f"out[{bssn.EVOL_GFS.index(g)} * Ntot + idx]" for g, _ in sub.outputs]
             body = _ordered_body(sub, fd_order, lv, rats, cse_scope=scope, linear=linear, prefetch=prefetch,
-                                 ahead=ahead)
+                                 ahead=ahead, u32=u32)
             has_up = any(d.op == "dupD" for d in sub.classify()[2])
-            upw = _upwind_prologue(sub, fd_order, rats, tables=(linear == "stream")) if has_up else ""
+            upw = _upwind_prologue(sub, fd_order, rats, tables=(linear == "stream"), u32=u32) if has_up else ""
         else:
             lv = [f"out[{bssn.AUXEVOL_GFS.index(g)} * Ntot + idx]" for g, _ in sub.outputs]
-            body = _ordered_body(sub, fd_order, lv, rats, cse_scope=scope, prefetch=prefetch, ahead=ahead)
+            body = _ordered_body(sub, fd_order, lv, rats, cse_scope=scope, prefetch=prefetch, ahead=ahead, u32=u32)
             upw = ""
         total += FLOPS[f"{sub.name}_fd{fd_order}{_mode_suffix(linear if kind == 'rhs' else 'stream', prefetch)}"]
         kernels.append(f"""
 extern "C" __global__ void __launch_bounds__({LAUNCH_BOUNDS[lb]})
 {kname}(const double *__restrict__ in, const double *__restrict__ aux, double *__restrict__ out,
         const double *__restrict__ rfm0, const double *__restrict__ rfm1, const bhah_kparams p) {{
-{_prologue(True, xcd)}
+{_prologue(True, xcd, u32)}
 {upw}
 {body}
 }}
@@ -784,10 +815,13 @@ extern "C" __global__ void __launch_bounds__({LAUNCH_BOUNDS[lb]})
         calls.append(f"  hipLaunchKernelGGL({kname}, g, b, 0, s, in, aux, out, rfm0, rfm1, p);")
         if ensemble:
             ename = f"bssn_ens_{kind}_{tag}_{cand}_g{k}"
-            ens_kernels.append(_ensemble_kernel(ename, upw, body, LAUNCH_BOUNDS[lb]))
+            ens_kernels.append(_ensemble_kernel(ename, upw, body, LAUNCH_BOUNDS[lb], u32))
             ens_calls.append(f"  hipLaunchKernelGGL({ename}, g, b, 0, s, mem, items, nitems, in_sel, aux_sel, out_sel);")
     consts = "\n".join(f"#define {k} ({v})" for k, v in sorted(rats.items()))
     undefs = "\n".join(f"#undef {k}" for k in sorted(rats))
+    if u32:
+        consts = BLD_MACRO + consts
+        undefs += "\n#undef BLD"
     launcher = f"""
 extern "C" hipError_t launch_bssn_{kind}_{tag}_{cand}({CAND_SIG}) {{
 {chr(10).join(calls)}

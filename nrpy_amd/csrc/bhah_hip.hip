@@ -1048,6 +1048,9 @@ int bhah_hip_patch_select_schedule(bhah_patch *P, int kind, int index) {
   const int n = bhah_hip_patch_num_schedules(P, kind);
   if (n < 0) return n;
   if (index < 0 || index >= n) return fail(BHAH_ERR_ARG, "schedule index %d out of range [0,%d)", index, n);
+  const char *nm = (kind == BHAH_KERNEL_RHS ? P->k->rhs : P->k->ricci)[index].name;
+  if (strstr(nm, "_u32") && P->Ntot * 24 * 8 >= (1L << 31))
+    return fail(BHAH_ERR_UNSUPPORTED, "schedule %s addresses arrays below 2 GiB only", nm);
   if (kind == BHAH_KERNEL_RHS) P->rhs_choice = index;
   else P->ricci_choice = index;
   P->invalidate_graph();
@@ -1466,6 +1469,9 @@ int bhah_hip_ensemble_select_schedule(bhah_ensemble *E, int kind, int index) {
   if (index < 0 || index >= n) return fail(BHAH_ERR_ARG, "schedule index %d out of range [0,%d)", index, n);
   const BssnCandidate &c = (kind == BHAH_KERNEL_RHS ? E->k->rhs : E->k->ricci)[index];
   if (!c.ens) return fail(BHAH_ERR_UNSUPPORTED, "schedule %s has no batched form", c.name);
+  for (const bhah_patch *P : E->P)
+    if (strstr(c.name, "_u32") && P->Ntot * 24 * 8 >= (1L << 31))
+      return fail(BHAH_ERR_UNSUPPORTED, "schedule %s addresses arrays below 2 GiB only", c.name);
   if (kind == BHAH_KERNEL_RHS) E->rhs_choice = index;
   else E->ricci_choice = index;
   E->invalidate_graph();

@@ -22,6 +22,22 @@ inline hipError_t hipGetLastError() { return hipSuccess; }
 #define __launch_bounds__(...)
 #define __builtin_amdgcn_sched_barrier(x) ((void)0)
 
+// Buffer resources: a base pointer and a byte size; loads outside it abort.
+struct __amdgpu_buffer_rsrc_t {
+  const char *base;
+  unsigned long bytes;
+};
+inline __amdgpu_buffer_rsrc_t __builtin_amdgcn_make_buffer_rsrc(void *p, short, int n, int) {
+  return {(const char *)p, (unsigned long)(unsigned)n};
+}
+inline double __builtin_amdgcn_raw_buffer_load_b64(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so, int) {
+  const unsigned long off = (unsigned long)vo + so;
+  if (r.base == nullptr || off + 8 > r.bytes) __builtin_trap();
+  double v;
+  __builtin_memcpy(&v, r.base + off, 8);
+  return v;
+}
+
 template <typename K, typename... A>
 void cpu_launch(K kernel, dim3 g, dim3 b, A... args) {
   gridDim = g;
