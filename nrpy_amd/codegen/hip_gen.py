@@ -4,22 +4,21 @@ Kernels are generated from this repository's SymPy transcription
 (``nrpy_amd.symbolic``) with FP64 arithmetic, no MFMA (stencil work is not a
 dense contraction).  One thread owns one gridpoint; a 64-lane wavefront spans
 64 consecutive xx0 points, so every stencil load of a wavefront is one
-coalesced 512-byte row.  Design choices, per kernel family:
+coalesced 512-byte row.
 
-* Interior BSSN RHS (``rhs_eval``, ``nrpy/infrastructures/BHaH/general_relativity/rhs_eval.py:336-379``):
-  centre values, stencil sums and the CSE'd algebra are emitted in
-  first-use order so the register allocator sees short live ranges.
-  Upwinded derivatives use one weight vector per direction, selected once per
-  point from the sign of beta^i and shared by all gridfunctions, instead of two
-  one-sided stencils blended afterwards (``nrpy/c_codegen.py:1085-1150``); the
-  result is the same derivative up to rounding.
-* Fused epilogue: the RK4 substep update
-  (``MoLtimestepping/tests/register_all_MoL_step_forward_in_time.c:4-103``) and
-  det(gammabar) enforcement (``enforce_detgammabar_equals_detgammahat.py:84-98``)
-  are applied in registers before the single store of the next stage; the raw
-  RHS is kept only at points that boundary conditions read.
-* Ricci (``Ricci_eval.py:95-142``) is a separate interior kernel writing the 6
-  auxiliary gridfunctions.
+* Interior BSSN right-hand side (``rhs_eval``,
+  ``nrpy/infrastructures/BHaH/general_relativity/rhs_eval.py:336-379``) and
+  Ricci tensor (``Ricci_eval.py:95-142``) come as sets of candidate schedules
+  (``candidate_tu``): output partitions, register budgets, CSE over expanded
+  or staged algebra, shared upwind/KO line loads, software-pipelined stencil
+  loads and buffer-resource addressing; an on-device autotuner picks one per
+  workload.  Upwinded derivatives select one-sided stencils per direction from
+  the sign of beta^i (``nrpy/c_codegen.py:1085-1150``).
+* The RK4 substep update (``MoLtimestepping/tests/register_all_MoL_step_forward_in_time.c:4-103``)
+  and det(gammabar) enforcement (``enforce_detgammabar_equals_detgammahat.py:84-98``)
+  run as one streaming pass over all points after the boundary conditions.
+* Ensemble builds also get persistent batched forms of every schedule and of
+  the RK pass (``_ensemble_kernel``).
 """
 
 from __future__ import annotations
