@@ -44,18 +44,15 @@ PEAK_HBM_GBS = 8000.0
 # the same counters on a plain 512 MiB device copy (FETCH_SIZE x2.000, WRITE_SIZE
 # x1.000); summary and commands in profiles/bssn_sinhspherical_192x96x96_hbm_traffic.md.
 MEASURED_TRAFFIC = {
+    ("SinhSpherical_symnone_KO1_fd6", (192, 96, 96), "p2_lb1_sj_ln_pf16_u32"): {
+        "fetch_bytes": 2 * (1414830 + 1087680) * 1024.0,
+        "write_bytes": (207360 + 373248) * 1024.0,
+    },
     ("SinhSpherical_symnone_KO1_fd6", (192, 96, 96), "p3_lb1_sl"): {
         "fetch_bytes": 2 * (625803 + 833376 + 1026360) * 1024.0,
         "write_bytes": (152064 + 96768 + 387072) * 1024.0,
     },
 }
-
-
-# Batched ensemble: 64 independent head-on Brill-Lindquist evolutions on Spherical grids drawn
-# uniformly from four sizes, masses m1, m2 ~ U[0.3, 0.7], separation d ~ U[0.6, 2.0] along z
-# (punctures at z = +-d/2), rmax = 10, 4th-order FD, RK4, each member at its own CFL timestep.
-C4_GRIDS = [(48, 8, 2), (72, 12, 2), (96, 16, 4), (128, 24, 8)]
-C4_SEED = 20261202
 
 
 def c4_members(n: int = 64, seed: int = C4_SEED):
