@@ -12,8 +12,9 @@ with all fields resident in HBM.  Prints one JSON line.
 
 The line also carries "batched": the 64-member ensemble (seeded masses, separations and
 grids) through the batched path against the same members stepped one after another
-with plain per-patch launches, and "adjoint": the lapse-loss gradient (checkpointed
-forward plus reverse pass) against the plain forward run.
+with plain per-patch launches, "adjoint": the lapse-loss gradient (checkpointed
+forward plus reverse pass) against the plain forward run, and "wave_c1" / "bssn_c2":
+the scalar wave on 64^3 and the BSSN head-on collision on 72x12x2.
     python bench.py --copy-calibration 512   (plain 512 MiB device copy x3, for
                                               scaling the FETCH_SIZE/WRITE_SIZE counters)
 """
@@ -53,6 +54,12 @@ MEASURED_TRAFFIC = {
         "write_bytes": (152064 + 96768 + 387072) * 1024.0,
     },
 }
+
+# Batched ensemble: 64 independent head-on Brill-Lindquist evolutions on Spherical grids drawn
+# uniformly from four sizes, masses m1, m2 ~ U[0.3, 0.7], separation d ~ U[0.6, 2.0] along z
+# (punctures at z = +-d/2), rmax = 10, 4th-order FD, RK4, each member at its own CFL timestep.
+C4_GRIDS = [(48, 8, 2), (72, 12, 2), (96, 16, 4), (128, 24, 8)]
+C4_SEED = 20261202
 
 
 def c4_members(n: int = 64, seed: int = C4_SEED):
@@ -140,6 +147,67 @@ def bench_batched(steps: int, warmup: int, n: int = 64) -> dict:
     }
 
 
+def _timed(fn):
+    import torch
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fn()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def bench_wave(steps: int = 200) -> dict:
+    """Scalar wave, Cartesian 64^3, 4th-order FD, RK4 at CFL 0.5 on [-10,10]^3 (200 steps, as the
+    reference example); uu = 2 + Gaussian pulse, vv = 0; error against the exact solution
+    inside r < 5 at the end, where the extrapolation boundary has not reached."""
+    import numpy as np
+    import torch
+
+    from nrpy_amd import initial_data as ID
+    from nrpy_amd.runtime import CARTESIAN, SYSTEM_WAVE, MoLState, Patch, PatchConfig
+
+    P = Patch(PatchConfig(system=SYSTEM_WAVE, coord=CARTESIAN, Nxx=(64, 64, 64), fd_order=4, symmetry_axes=0,
+                          grid_physical_size=10.0, wavespeed=1.0))
+    xx = P.xx()
+    y0 = ID.wave_spherical_gaussian(xx, 0.0, sigma=1.0)
+    S = MoLState(P, torch.from_numpy(np.ascontiguousarray(y0)).cuda())
+    S.steps(2, P.dt)
+    S.y_n.copy_(torch.from_numpy(np.ascontiguousarray(y0)).cuda())
+    S.nn = 0
+    sec = _timed(lambda: S.steps(steps, P.dt))
+    exact = ID.wave_spherical_gaussian(xx, steps * P.dt, sigma=1.0)
+    X, Y, Z = ID.cartesian_coords(0, xx, 0.0, 0.0)
+    mask = np.sqrt(X * X + Y * Y + Z * Z) < 5.0
+    err = float(np.abs(S.y_n.cpu().numpy()[0] - exact[0])[mask].max())
+    npts = 64 ** 3
+    return {"workload": "wave_cartesian_64^3_fd4_rk4_200steps", "value": npts * steps / sec,
+            "unit": "gridpoint-RK4-steps/s", "ms_per_step": sec * 1e3 / steps,
+            "hbm_GBs_at_272B_per_point_step": 272.0 * npts * steps / sec / 1e9, "max_err_r_lt_5": err}
+
+
+def bench_c2(steps: int = 200) -> dict:
+    """BSSN head-on collision on Spherical 72x12x2 (C2), hipGraph steps: a 1,728-point grid,
+    bound by kernel launch latency rather than arithmetic or bandwidth."""
+    import numpy as np
+    import torch
+
+    from nrpy_amd import initial_data as ID
+    from nrpy_amd.runtime import MoLState, Patch, PatchConfig
+
+    c = CONFIGS["c2"]
+    P = Patch(PatchConfig(coord=c["coord"], Nxx=c["Nxx"], fd_order=4, symmetry_axes=4, grid_physical_size=c["size"],
+                          eta=2.0))
+    y = ID.brill_lindquist(1, P.xx(), c["size"], 0.2, c["masses"], c["pos1"], c["pos2"])
+    S = MoLState(P, torch.from_numpy(np.ascontiguousarray(y)).cuda())
+    P.apply_bcs_outerextrap_and_inner(S.y_n)
+    S.steps(2, P.dt)
+    sec = _timed(lambda: S.steps(steps, P.dt))
+    npts = c["Nxx"][0] * c["Nxx"][1] * c["Nxx"][2]
+    return {"workload": c["workload"], "value": npts * steps / sec, "unit": "gridpoint-RK4-steps/s",
+            "us_per_step": sec * 1e6 / steps}
+
+
 def bench_adjoint() -> dict:
     """Lapse-extraction loss on Spherical 96x16x4 (400 RK4 steps, loss over the last 50,
     16 samples on r = 4, checkpoints every 20): time of the plain forward run against
@@ -218,6 +286,7 @@ def main() -> None:
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-ensemble comparison")
     ap.add_argument("--batched-steps", type=int, default=200)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient timing")
+    ap.add_argument("--no-small", action="store_true", help="skip the wave 64^3 and BSSN 72x12x2 lines")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB three times, then exit")
     args = ap.parse_args()
@@ -342,6 +411,9 @@ def main() -> None:
             "ricci_tflops": flops_ricci * npts / (ricci_ms * 1e-3) / 1e12,
         },
     }
+    if not args.no_small:
+        result["wave_c1"] = bench_wave()
+        result["bssn_c2"] = bench_c2()
     if not args.no_batched:
         result["batched"] = bench_batched(args.batched_steps, 2)
     if not args.no_adjoint:
