@@ -8,15 +8,16 @@ data with masses 0.6 and 0.4 at (0.3,0.2,0.5) and (-0.3,-0.2,-0.5).
 value = interior points x timed RK4 steps / seconds of MoL_step_forward_in_time,
 with all fields resident in HBM.  Prints one JSON line.
 
-    python bench.py [--gpus 1] [--steps 100] [--warmup 5] [--config c3|c2] [--no-cpu] [--no-batched]
+    python bench.py [--gpus 1] [--steps 100] [--warmup 5] [--config c3|c2] [--no-cpu]
+                    [--no-batched] [--no-adjoint] [--no-small]
+    python bench.py --copy-calibration 512   (plain 512 MiB device copy x3, for
+                                              scaling the FETCH_SIZE/WRITE_SIZE counters)
 
 The line also carries "batched": the 64-member ensemble (seeded masses, separations and
 grids) through the batched path against the same members stepped one after another
 with plain per-patch launches, "adjoint": the lapse-loss gradient (checkpointed
 forward plus reverse pass) against the plain forward run, and "wave_c1" / "bssn_c2":
 the scalar wave on 64^3 and the BSSN head-on collision on 72x12x2.
-    python bench.py --copy-calibration 512   (plain 512 MiB device copy x3, for
-                                              scaling the FETCH_SIZE/WRITE_SIZE counters)
 """
 
 from __future__ import annotations
@@ -160,7 +161,7 @@ def _timed(fn):
 def bench_wave(steps: int = 200) -> dict:
     """Scalar wave, Cartesian 64^3, 4th-order FD, RK4 at CFL 0.5 on [-10,10]^3 (200 steps, as the
     reference example); uu = 2 + Gaussian pulse, vv = 0; max error against the exact
-    solution inside r < 5 at the end (by then boundary reflections dominate it).""" 
+    solution inside r < 5 at the end (by then boundary reflections dominate it)."""
     import numpy as np
     import torch
 
