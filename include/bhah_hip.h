@@ -183,6 +183,25 @@ int bhah_hip_wave_apply_bcs(bhah_patch *patch, double *gfs);
 int bhah_hip_diagnostics_constraints(bhah_patch *patch, const double *auxevol_gfs, const double *in_gfs, double *diag,
                                      double *l2_H);
 
+/* Checkpoints (write_checkpoint / read_checkpoint, BHaH/tests/write_checkpoint_write_checkpoint__openmp.c:84-117,
+ * read_checkpoint_read_checkpoint__openmp.c:64-160).  The reference's file is
+ * commondata_struct, then per grid params_struct followed by this payload:
+ * int count; int indices[count] (strictly increasing); double data[count * NUM_EVOL_GFS],
+ * point-major (data[e * NUM_EVOL_GFS + gf] = y_n[gf * Ntot + indices[e]]).  The structs are
+ * project-generated, so the caller writes them; these calls produce and consume the
+ * payload.  Pack lists every point; unpack validates like the reference reader and
+ * scatters the listed points into y_n (device).  Both synchronise the patch stream. */
+int64_t bhah_hip_checkpoint_payload_bytes(const bhah_patch *patch);
+int bhah_hip_checkpoint_pack(bhah_patch *patch, const double *y_n, void *out);
+int bhah_hip_checkpoint_unpack(bhah_patch *patch, const void *payload, int64_t bytes, double *y_n);
+
+/* Brill-Lindquist initial data on the device at every point, ghost zones included
+ * (InitialData_Cartesian BrillLindquist with the ADM-to-BSSN conversion,
+ * ADM_Initial_Data_Reader__BSSN_Converter.py:268-580): the data are conformally flat, so
+ * gammabar = gammahat, h_ij = a_ij = lambda^i = 0 and K = beta = B = 0; cf = alpha = psi^-2 with
+ * psi = 1 + m1 / (2 |x - x1|) + m2 / (2 |x - x2|).  bh = {m1, x1, y1, z1, m2, x2, y2, z2}. */
+int bhah_hip_initial_data_brill_lindquist(bhah_patch *patch, const double bh[8], double *y_n);
+
 /* Reverse mode of MoL_step_forward_in_time (one RK4 step; no reference counterpart --
  * the reference has no adjoint).  Given y_n, the state at the start of the step, and
  * lam, the adjoint (gradient of a scalar loss) with respect to the state after the

@@ -382,6 +382,60 @@ axpy_kernel(double *__restrict__ dst, const double *__restrict__ src, double a, 
   if (t < n) dst[t] += a * src[t];
 }
 
+// ---- Checkpoint payload (write_checkpoint__openmp.c:95-117): point-major copy of y_n.
+__global__ void __launch_bounds__(256)
+checkpoint_pack_kernel(const double *__restrict__ y, double *__restrict__ compact, long ntot, int ngf) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntot * ngf) return;
+  const long p = t / ngf;
+  const int g = (int)(t - p * ngf);
+  compact[t] = y[g * ntot + p];
+}
+__global__ void __launch_bounds__(256)
+checkpoint_unpack_kernel(const int *__restrict__ idx, const double *__restrict__ compact, long count, int ngf,
+                         long ntot, double *__restrict__ y) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count * ngf) return;
+  const long e = t / ngf;
+  const int g = (int)(t - e * ngf);
+  y[g * ntot + idx[e]] = compact[t];
+}
+
+// ---- Brill-Lindquist initial data on every point (ghost zones included): conformal
+// factor W = psi^-2 and lapse psi^-2, all other BSSN variables zero
+// (InitialData_Cartesian BrillLindquist; conformally flat, so gammabar = gammahat and the
+// rescaled h_ij, lambda^i vanish).
+__global__ void __launch_bounds__(256)
+initial_data_bl_kernel(double *__restrict__ y, int coord, double AMPL, double SINHW, int Nx0, int Nx1, int Nx2, int NG,
+                       double xmin0, double xmin1, double xmin2, double dx0, double dx1, double dx2, double m1,
+                       double x1, double y1, double z1, double m2, double x2, double y2, double z2, long ntot) {
+#pragma clang fp contract(off)
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= ntot) return;
+  const int i0 = (int)(p % Nx0), i1 = (int)((p / Nx0) % Nx1), i2 = (int)(p / ((long)Nx0 * Nx1));
+  const double q0 = xmin0 + ((double)(i0 - NG) + (1.0 / 2.0)) * dx0;
+  const double q1 = xmin1 + ((double)(i1 - NG) + (1.0 / 2.0)) * dx1;
+  const double q2 = xmin2 + ((double)(i2 - NG) + (1.0 / 2.0)) * dx2;
+  double X, Y, Z;
+  if (coord == BHAH_CARTESIAN) {
+    X = q0;
+    Y = q1;
+    Z = q2;
+  } else {
+    double r = q0;
+    if (coord == BHAH_SINHSPHERICAL)  // reference_metric.py:1224-1235, exponential form
+      r = AMPL * (exp(q0 / SINHW) - exp(-q0 / SINHW)) / (exp(1.0 / SINHW) - exp(-1.0 / SINHW));
+    X = r * sin(q1) * cos(q2);
+    Y = r * sin(q1) * sin(q2);
+    Z = r * cos(q1);
+  }
+  const double r1 = sqrt((X - x1) * (X - x1) + (Y - y1) * (Y - y1) + (Z - z1) * (Z - z1));
+  const double r2 = sqrt((X - x2) * (X - x2) + (Y - y2) * (Y - y2) + (Z - z2) * (Z - z2));
+  const double psi = 1.0 + m1 / (2.0 * r1) + m2 / (2.0 * r2);
+  const double w = 1.0 / (psi * psi);
+  for (int g = 0; g < BHAH_NUM_EVOL_GFS_BSSN; g++) y[g * ntot + p] = (g == 6 || g == 10) ? w : 0.0;
+}
+
 // Volume-weighted sums over the interior for the constraint norms.
 __global__ void __launch_bounds__(256)
 l2_reduce_kernel(const double *__restrict__ h, const double *__restrict__ msq, const double *__restrict__ rfm0,
@@ -1106,6 +1160,87 @@ int bhah_hip_diagnostics_constraints(bhah_patch *P, const double *aux, const dou
     l2_H[1] = sqrt(red[1]);           // sqrt of the volume integral of H^2
     l2_H[2] = red[0];                 // proper volume
   }
+  return BHAH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Checkpoints in the reference's per-grid payload layout, and initial data on the device.
+
+extern "C" int64_t bhah_hip_checkpoint_payload_bytes(const bhah_patch *P) {
+  if (!P) return fail(BHAH_ERR_ARG, "NULL patch");
+  return (int64_t)sizeof(int) * (1 + P->Ntot) + (int64_t)sizeof(double) * P->Ntot * P->ngfs;
+}
+
+extern "C" int bhah_hip_checkpoint_pack(bhah_patch *P, const double *y_n, void *out) {
+  if (!P || !y_n || !out) return fail(BHAH_ERR_ARG, "NULL argument");
+  if (P->Ntot > 0x7fffffffL) return fail(BHAH_ERR_ARG, "grid has more points than an int index holds");
+  const long n = P->Ntot, nv = n * P->ngfs;
+  char *o = (char *)out;
+  const int count = (int)n;
+  memcpy(o, &count, sizeof(int));
+  int *ind = (int *)(o + sizeof(int));
+  for (long i = 0; i < n; i++) ind[i] = (int)i;
+  double *dtmp = nullptr;
+  HIP_TRY(hipMallocAsync((void **)&dtmp, nv * sizeof(double), P->stream));
+  dim3 g, b;
+  launch_1d(nv, g, b);
+  hipLaunchKernelGGL(checkpoint_pack_kernel, g, b, 0, P->stream, y_n, dtmp, n, P->ngfs);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(o + sizeof(int) * (1 + n), dtmp, nv * sizeof(double), hipMemcpyDeviceToHost,
+                                          P->stream);
+  hipFreeAsync(dtmp, P->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(P->stream);
+  if (e != hipSuccess) return fail(BHAH_ERR_HIP, "checkpoint pack failed: %s", hipGetErrorString(e));
+  return BHAH_OK;
+}
+
+extern "C" int bhah_hip_checkpoint_unpack(bhah_patch *P, const void *payload, int64_t bytes, double *y_n) {
+  if (!P || !payload || !y_n) return fail(BHAH_ERR_ARG, "NULL argument");
+  const char *c = (const char *)payload;
+  if (bytes < (int64_t)sizeof(int)) return fail(BHAH_ERR_ARG, "checkpoint payload truncated");
+  int count;
+  memcpy(&count, c, sizeof(int));
+  if (count < 0 || count > P->Ntot) return fail(BHAH_ERR_ARG, "invalid count=%d (ntot=%ld)", count, P->Ntot);
+  const int64_t need = (int64_t)sizeof(int) * (1 + count) + (int64_t)sizeof(double) * count * P->ngfs;
+  if (bytes < need) return fail(BHAH_ERR_ARG, "checkpoint payload truncated (%lld < %lld bytes)", (long long)bytes,
+                                (long long)need);
+  const int *ind = (const int *)(c + sizeof(int));
+  for (int i = 0; i < count; i++) {
+    if (ind[i] < 0 || ind[i] >= P->Ntot) return fail(BHAH_ERR_ARG, "invalid gridpoint index %d at element %d", ind[i], i);
+    if (i > 0 && ind[i] <= ind[i - 1]) return fail(BHAH_ERR_ARG, "non-increasing gridpoint index at element %d", i);
+  }
+  if (!count) return BHAH_OK;
+  const long nv = (long)count * P->ngfs;
+  int *dind = nullptr;
+  double *ddat = nullptr;
+  HIP_TRY(hipMallocAsync((void **)&dind, count * sizeof(int), P->stream));
+  HIP_TRY(hipMallocAsync((void **)&ddat, nv * sizeof(double), P->stream));
+  hipError_t e = hipMemcpyAsync(dind, ind, count * sizeof(int), hipMemcpyHostToDevice, P->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(ddat, c + sizeof(int) * (1 + (long)count), nv * sizeof(double), hipMemcpyHostToDevice,
+                       P->stream);
+  if (e == hipSuccess) {
+    dim3 g, b;
+    launch_1d(nv, g, b);
+    hipLaunchKernelGGL(checkpoint_unpack_kernel, g, b, 0, P->stream, dind, ddat, (long)count, P->ngfs, P->Ntot, y_n);
+    e = hipGetLastError();
+  }
+  hipFreeAsync(dind, P->stream);
+  hipFreeAsync(ddat, P->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(P->stream);
+  if (e != hipSuccess) return fail(BHAH_ERR_HIP, "checkpoint unpack failed: %s", hipGetErrorString(e));
+  return BHAH_OK;
+}
+
+extern "C" int bhah_hip_initial_data_brill_lindquist(bhah_patch *P, const double bh[8], double *y_n) {
+  if (!P || !bh || !y_n) return fail(BHAH_ERR_ARG, "NULL argument");
+  if (P->d.system != BHAH_SYSTEM_BSSN) return fail(BHAH_ERR_ARG, "patch is not a BSSN patch");
+  dim3 g, b;
+  launch_1d(P->Ntot, g, b);
+  hipLaunchKernelGGL(initial_data_bl_kernel, g, b, 0, P->stream, y_n, P->d.coord, P->d.grid_physical_size, P->d.SINHW,
+                     P->Nx[0], P->Nx[1], P->Nx[2], P->NG, P->xxmin[0], P->xxmin[1], P->xxmin[2], P->dxx[0], P->dxx[1],
+                     P->dxx[2], bh[0], bh[1], bh[2], bh[3], bh[4], bh[5], bh[6], bh[7], P->Ntot);
+  HIP_TRY(hipGetLastError());
   return BHAH_OK;
 }
 

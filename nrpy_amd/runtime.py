@@ -117,6 +117,10 @@ SIGNATURES: Dict[str, tuple] = {
     "bhah_hip_wave_apply_bcs": (_I, [_P, _P]),
     "bhah_hip_diagnostics_constraints": (_I, [_P, _P, _P, _P, ctypes.POINTER(_D)]),
     "bhah_hip_MoL_step_adjoint": (_I, [_P, _P, _D, _P]),
+    "bhah_hip_checkpoint_payload_bytes": (ctypes.c_int64, [_P]),
+    "bhah_hip_checkpoint_pack": (_I, [_P, _P, _P]),
+    "bhah_hip_checkpoint_unpack": (_I, [_P, _P, ctypes.c_int64, _P]),
+    "bhah_hip_initial_data_brill_lindquist": (_I, [_P, ctypes.POINTER(_D), _P]),
     "bhah_hip_ensemble_create": (_I, [ctypes.POINTER(PatchDesc), _I, ctypes.POINTER(_P)]),
     "bhah_hip_ensemble_destroy": (_I, [_P]),
     "bhah_hip_ensemble_size": (_I, [_P]),
@@ -340,6 +344,25 @@ class Patch:
 
     def synchronize(self):
         _check(self.lib.bhah_hip_synchronize(self.handle))
+
+    def checkpoint_payload(self, y_n) -> bytes:
+        """The reference checkpoint's per-grid payload (after params_struct) for y_n."""
+        n = self.lib.bhah_hip_checkpoint_payload_bytes(self.handle)
+        if n < 0:
+            _check(int(n))
+        buf = ctypes.create_string_buffer(int(n))
+        _check(self.lib.bhah_hip_checkpoint_pack(self.handle, _ptr(y_n), buf))
+        return buf.raw
+
+    def restore_payload(self, payload: bytes, y_n) -> int:
+        """Scatter a per-grid checkpoint payload into y_n; returns the bytes consumed."""
+        _check(self.lib.bhah_hip_checkpoint_unpack(self.handle, payload, len(payload), _ptr(y_n)))
+        count = int.from_bytes(payload[:4], "little", signed=True)
+        return 4 * (1 + count) + 8 * count * self.ngfs
+
+    def initial_data_brill_lindquist(self, y_n, m1, pos1, m2, pos2):
+        bh = (ctypes.c_double * 8)(m1, *pos1, m2, *pos2)
+        _check(self.lib.bhah_hip_initial_data_brill_lindquist(self.handle, bh, _ptr(y_n)))
 
     def MoL_step_adjoint(self, y_n, dt: float, lam):
         """lam (adjoint of the state after one RK4 step from y_n) <- adjoint of y_n."""

@@ -666,3 +666,38 @@ double og_cfl_dt(const og_grid *g, const double *x0, const double *x1, const dou
       }
   return ds_min * CFL;
 }
+
+/* ---- Checkpoint payload (after params_struct), as write_checkpoint writes it
+ * (BHaH/tests/write_checkpoint_write_checkpoint__openmp.c:84-117): int count, int indices[count]
+ * (every point, increasing), then REAL compact[count * ngf] with compact[el * ngf + gf] =
+ * y[ntot * gf + indices[el]].  Returns bytes written. */
+long og_checkpoint_pack(long ntot, int ngf, const double *y, void *out) {
+  char *o = (char *)out;
+  const int count = (int)ntot;
+  memcpy(o, &count, sizeof(int));
+  int *ind = (int *)(o + sizeof(int));
+  double *compact = (double *)(o + sizeof(int) * (1 + ntot));
+  long which_el = 0;
+  for (long i = 0; i < ntot; i++) {
+    ind[which_el] = (int)i;
+    for (int gf = 0; gf < ngf; gf++) compact[which_el * ngf + gf] = y[ntot * gf + i];
+    which_el++;
+  }
+  return (long)sizeof(int) * (1 + ntot) + (long)sizeof(double) * ntot * ngf;
+}
+
+/* Inverse, as read_checkpoint fills y_n (read_checkpoint_read_checkpoint__openmp.c:64-160):
+ * returns 0, or -1 on an invalid count or index (the reference exits). */
+int og_checkpoint_unpack(long ntot, int ngf, const void *in, double *y) {
+  const char *c = (const char *)in;
+  int count;
+  memcpy(&count, c, sizeof(int));
+  if (count < 0 || count > ntot) return -1;
+  const int *ind = (const int *)(c + sizeof(int));
+  const double *compact = (const double *)(c + sizeof(int) * (1 + (long)count));
+  for (int i = 0; i < count; i++) {
+    if (ind[i] < 0 || ind[i] >= ntot || (i > 0 && ind[i] <= ind[i - 1])) return -1;
+    for (int gf = 0; gf < ngf; gf++) y[ntot * gf + ind[i]] = compact[(long)i * ngf + gf];
+  }
+  return 0;
+}

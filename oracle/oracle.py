@@ -74,6 +74,8 @@ def lib():
             "og_bssn_mol_step": (None, [P, I, P, P, P, P, P, I, D, P, P, P, P, P]),
             "og_wave_mol_step": (None, [P, D, D, P, P, P, P]),
             "og_cfl_dt": (D, [P, P, P, P, D]),
+            "og_checkpoint_pack": (ctypes.c_long, [ctypes.c_long, I, P, P]),
+            "og_checkpoint_unpack": (I, [ctypes.c_long, I, P, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(_lib, name)
