@@ -264,7 +264,7 @@ def _emit_defs(spec: specs.Spec, fd_order: int, u32: bool = False):
 def _ordered_body(spec: specs.Spec, fd_order: int, out_lvalues: List[str], rationals: Dict[str, str],
                   stream_linear: bool = True, order: str = "dfs", barriers: bool = True,
                   cse_scope: str = "joint", linear: str = "stream", prefetch: int = 0,
-                  ahead: int = 12, u32: bool = False) -> str:
+                  ahead: int = 12, u32: bool = False, chunk: int = 0) -> str:
     """Stencils and CSE'd algebra for one pointwise spec.
 
     With ``stream_linear``, upwind and KO derivatives are consumed as soon as
@@ -477,7 +477,7 @@ def _ordered_body(spec: specs.Spec, fd_order: int, out_lvalues: List[str], ratio
         flops[0] += emit._flops(e) + (1 if tail else 0)
     FLOPS[f"{spec.name}_fd{fd_order}{_mode_suffix(linear, prefetch)}"] = flops[0]
     if prefetch:
-        items = _schedule_prefetch(items, prefetch, ahead)
+        items = _schedule_prefetch(items, prefetch, ahead, chunk)
     out = []
     for it in items:
         out.append("  __builtin_amdgcn_sched_barrier(0);" if it[0] == "B" else it[1])
@@ -492,13 +492,16 @@ def _mode_suffix(linear: str, prefetch: int) -> str:
     return ("" if linear == "stream" else f"_{linear}") + (f"_pf{prefetch}" if prefetch else "")
 
 
-def _schedule_prefetch(items: List[tuple], budget: int, ahead: int) -> List[tuple]:
+def _schedule_prefetch(items: List[tuple], budget: int, ahead: int, chunk: int = 0) -> List[tuple]:
     """Move each group of loads to the earliest region allowed by the budget.
 
     Regions are the stretches between scheduling barriers.  Groups keep their
     consumption order (so a wait for one group never waits for a later one);
     a group consumed in region c may start in region s >= c - ahead when every
     region in [s, c) holds fewer than ``budget`` early values including it.
+    With ``chunk``, groups are first cut into pieces of at most that many loads,
+    so stencils wider than the budget (mixed second derivatives) are issued
+    early piece by piece instead of all at their use.
     """
     regions: List[List[tuple]] = [[]]
     for it in items:
@@ -526,6 +529,14 @@ def _schedule_prefetch(items: List[tuple], budget: int, ahead: int) -> List[tupl
         # a region's own loads are issued first, then the early loads of later
         # regions, then its code: issue order equals consumption order
         placed_here: List[tuple] = []
+        if chunk:
+            pieces: Dict[tuple, List[tuple]] = {}
+            order_p: List[tuple] = []
+            for g in order_g:
+                for k in range(0, len(groups[g]), chunk):
+                    pieces[(g, k)] = groups[g][k:k + chunk]
+                    order_p.append((g, k))
+            groups, order_g = pieces, order_p
         for g in order_g:
             n = len(groups[g])
             sreg = c
@@ -738,16 +749,16 @@ LAUNCH_BOUNDS = {"lb1": "256", "lb2": "256, 2"}
 
 # Candidate schedules per build: (kind, partition, launch bounds).
 FULL_CANDIDATES = {
-    "rhs": ["p1_lb1", "p3_lb1_sl", "p3_lb1_sj_ln_pf16", "p3_lb1_sl_ln_pf16", "p3_lb1_sj_ln_pf12_u32",
-            "p3_lb1_sj_ln_pf16_u32", "p3_lb1_sj_ln_pf24_u32", "p3_lb1_sl_ln_pf16_u32", "p2_lb1_sj_ln_pf16_u32",
-            "p4_lb1_sj_ln_pf16_u32"],
-    "ricci": ["r1_lb1", "r3_lb1", "r2_lb1_sj_pf32", "r2_lb1_sl_pf24", "r2_lb1_sj_pf16_u32", "r2_lb1_sj_pf24_u32",
-              "r1_lb1_sj_pf24_u32"],
+    "rhs": ["p1_lb1", "p3_lb1_sj_ln_pf16", "p2_lb1_sj_ln_pf16_u32", "p3_lb1_sj_ln_pf16_u32",
+            "p2_lb1_sj_ln_pf16_u32_ck8", "p2_lb1_sj_ln_pf16_u32_ck16", "p3_lb1_sj_ln_pf16_u32_ck8",
+            "p2_lb1_sj_ln_pf12_u32_ck6", "p2_lb1_sj_ln_pf20_u32_ck10", "p2_lb1_sl_ln_pf16_u32_ck8"],
+    "ricci": ["r1_lb1", "r2_lb1_sj_pf32", "r2_lb1_sj_pf16_u32", "r2_lb1_sj_pf16_u32_ck8", "r2_lb1_sj_pf24_u32_ck8",
+              "r3_lb1_sj_pf24_u32_ck8", "r2_lb1_sj_pf20_u32_ck10", "r2_lb1_sj_pf32_ck16"],
 }
 SMALL_CANDIDATES = {"rhs": ["p1_lb1", "p1_lb1_sl", "p3_lb1_sl", "p1_lb1_sl_ln_pf16", "p3_lb1_sl_ln_pf16",
-                            "p1_lb1_sl_ln_pf16_u32", "p3_lb1_sl_ln_pf16_u32"],
+                            "p1_lb1_sl_ln_pf16_u32", "p3_lb1_sl_ln_pf16_u32", "p3_lb1_sl_ln_pf16_u32_ck8"],
                     "ricci": ["r1_lb1", "r1_lb1_sl", "r2_lb1_sl", "r1_lb1_pf16", "r2_lb1_sl_pf16", "r1_lb1_pf16_u32",
-                              "r2_lb1_sl_pf16_u32"]}
+                              "r2_lb1_sl_pf16_u32", "r2_lb1_sl_pf16_u32_ck8"]}
 
 CAND_SIG = ("dim3 g, dim3 b, hipStream_t s, const double *in, const double *aux, double *out, "
             "const double *rfm0, const double *rfm1, bhah_kparams p")
@@ -777,6 +788,7 @@ def candidate_tu(v: specs.Variant, fd_order: int, kind: str, cand: str, ensemble
     ahead = next((int(o[1:]) for o in opts if o[0] == "a" and o[1:].isdigit()), 12)
     xcd = "xcd" in opts
     u32 = "u32" in opts
+    chunk = next((int(o[2:]) for o in opts if o.startswith("ck") and o[2:].isdigit()), 0)
     rats: Dict[str, str] = {}
     kernels, calls, total = [], [], 0
     ens_kernels, ens_calls = [], []
@@ -794,12 +806,13 @@ def candidate_tu(v: specs.Variant, fd_order: int, kind: str, cand: str, ensemble
         if kind == "rhs":
             lv = [f"out[{bssn.EVOL_GFS.index(g)} * Ntot + idx]" for g, _ in sub.outputs]
             body = _ordered_body(sub, fd_order, lv, rats, cse_scope=scope, linear=linear, prefetch=prefetch,
-                                 ahead=ahead, u32=u32)
+                                 ahead=ahead, u32=u32, chunk=chunk)
             has_up = any(d.op == "dupD" for d in sub.classify()[2])
             upw = _upwind_prologue(sub, fd_order, rats, tables=(linear == "stream"), u32=u32) if has_up else ""
         else:
             lv = [f"out[{bssn.AUXEVOL_GFS.index(g)} * Ntot + idx]" for g, _ in sub.outputs]
-            body = _ordered_body(sub, fd_order, lv, rats, cse_scope=scope, prefetch=prefetch, ahead=ahead, u32=u32)
+            body = _ordered_body(sub, fd_order, lv, rats, cse_scope=scope, prefetch=prefetch, ahead=ahead, u32=u32,
+                                 chunk=chunk)
             upw = ""
         total += FLOPS[f"{sub.name}_fd{fd_order}{_mode_suffix(linear if kind == 'rhs' else 'stream', prefetch)}"]
         kernels.append(f"""
