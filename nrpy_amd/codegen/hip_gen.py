@@ -738,6 +738,7 @@ RHS_PARTITIONS = {
     "p2": [_A + ["alpha", "cf"] + _H + ["trK"], _L + _B + _V],
     "p3": [_H + ["cf", "alpha"] + _V, _A + ["trK"], _L + _B],
     "p4": [_H + ["cf", "alpha"] + _V, _A, ["trK"], _L + _B],
+    "p5": [_H + ["cf"], ["alpha"] + _V, _A, ["trK"], _L + _B],
 }
 RICCI_PARTITIONS = {
     "r1": [list(bssn.AUXEVOL_GFS)],
@@ -746,14 +747,22 @@ RICCI_PARTITIONS = {
     "r6": [[g] for g in bssn.AUXEVOL_GFS],
 }
 LAUNCH_BOUNDS = {"lb1": "256", "lb2": "256, 2"}
+# "lbm": two waves per SIMD for every group but the last (the heaviest: the Lambdabar^i and
+# B^i right-hand sides, or the last Ricci group), which keeps one wave and the full register file.
+
+
+def _group_bounds(lb: str, k: int, ngroups: int) -> str:
+    if lb == "lbm":
+        return LAUNCH_BOUNDS["lb1" if k == ngroups - 1 else "lb2"]
+    return LAUNCH_BOUNDS[lb]
 
 # Candidate schedules per build: (kind, partition, launch bounds).
 FULL_CANDIDATES = {
-    "rhs": ["p1_lb1", "p3_lb1_sj_ln_pf16", "p2_lb1_sj_ln_pf16_u32", "p3_lb1_sj_ln_pf16_u32",
-            "p2_lb1_sj_ln_pf16_u32_ck8", "p2_lb1_sj_ln_pf16_u32_ck16", "p3_lb1_sj_ln_pf16_u32_ck8",
-            "p2_lb1_sj_ln_pf12_u32_ck6", "p2_lb1_sj_ln_pf20_u32_ck10", "p2_lb1_sl_ln_pf16_u32_ck8"],
-    "ricci": ["r1_lb1", "r2_lb1_sj_pf32", "r2_lb1_sj_pf16_u32", "r2_lb1_sj_pf16_u32_ck8", "r2_lb1_sj_pf24_u32_ck8",
-              "r3_lb1_sj_pf24_u32_ck8", "r2_lb1_sj_pf20_u32_ck10", "r2_lb1_sj_pf32_ck16"],
+    "rhs": ["p1_lb1", "p2_lb1_sj_ln_pf16_u32", "p2_lb1_sj_ln_pf16_u32_ck8", "p2_lb1_sj_ln_pf20_u32_ck10",
+            "p3_lb1_sj_ln_pf16_u32", "p4_lbm_sj_ln_pf8_u32", "p4_lbm_sj_ln_pf12_u32", "p4_lbm_sl_ln_pf8_u32",
+            "p5_lbm_sj_ln_pf8_u32", "p5_lbm_sj_ln_pf12_u32", "p5_lbm_sj_ln_u32", "p5_lbm_sj_ln_pf4_u32"],
+    "ricci": ["r1_lb1", "r2_lb1_sj_pf32", "r2_lb1_sj_pf20_u32_ck10", "r2_lb1_sj_pf16_u32_ck8", "r6_lb2_sj_u32",
+              "r6_lb2_sj_pf4_u32", "r6_lb2_sj_pf8_u32", "r6_lb2_sl_pf8_u32"],
 }
 SMALL_CANDIDATES = {"rhs": ["p1_lb1", "p1_lb1_sl", "p3_lb1_sl", "p1_lb1_sl_ln_pf16", "p3_lb1_sl_ln_pf16",
                             "p1_lb1_sl_ln_pf16_u32", "p3_lb1_sl_ln_pf16_u32", "p3_lb1_sl_ln_pf16_u32_ck8"],
@@ -816,7 +825,7 @@ def candidate_tu(v: specs.Variant, fd_order: int, kind: str, cand: str, ensemble
             upw = ""
         total += FLOPS[f"{sub.name}_fd{fd_order}{_mode_suffix(linear if kind == 'rhs' else 'stream', prefetch)}"]
         kernels.append(f"""
-extern "C" __global__ void __launch_bounds__({LAUNCH_BOUNDS[lb]})
+extern "C" __global__ void __launch_bounds__({_group_bounds(lb, k, len(groups))})
 {kname}(const double *__restrict__ in, const double *__restrict__ aux, double *__restrict__ out,
         const double *__restrict__ rfm0, const double *__restrict__ rfm1, const bhah_kparams p) {{
 {_prologue(True, xcd, u32)}
@@ -827,7 +836,7 @@ extern "C" __global__ void __launch_bounds__({LAUNCH_BOUNDS[lb]})
         calls.append(f"  hipLaunchKernelGGL({kname}, g, b, 0, s, in, aux, out, rfm0, rfm1, p);")
         if ensemble:
             ename = f"bssn_ens_{kind}_{tag}_{cand}_g{k}"
-            ens_kernels.append(_ensemble_kernel(ename, upw, body, LAUNCH_BOUNDS[lb], u32))
+            ens_kernels.append(_ensemble_kernel(ename, upw, body, _group_bounds(lb, k, len(groups)), u32))
             ens_calls.append(f"  hipLaunchKernelGGL({ename}, g, b, 0, s, mem, items, nitems, in_sel, aux_sel, out_sel);")
     consts = "\n".join(f"#define {k} ({v})" for k, v in sorted(rats.items()))
     undefs = "\n".join(f"#undef {k}" for k in sorted(rats))
