@@ -633,9 +633,40 @@ extern "C" __global__ void __launch_bounds__({lb})
 """
 
 
-def _prologue(interior: bool, xcd: bool = False, u32: bool = False) -> str:
+def _zsweep_remap(sz: int) -> str:
+    """Workgroup order for L2 reuse along xx2: give XCD k the k-th contiguous eighth of a
+    logical tile order in which xx2 runs fastest within slabs of ``sz`` planes, so the
+    workgroups resident on one XCD at a time cover a few (xx0, xx1) tiles over ``sz``
+    consecutive planes and share their stencil neighbour planes in that XCD's L2.
+    Identity when the grid does not divide evenly."""
+    return f"""  unsigned bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+  {{
+    const unsigned gx = gridDim.x, gy = gridDim.y, gz = gridDim.z, n = gx * gy * gz;
+    if ((n & 7u) == 0u && gz % {sz}u == 0u) {{
+      const unsigned h = bx + gx * (by + gy * bz);
+      const unsigned l = (h & 7u) * (n >> 3) + (h >> 3);
+      const unsigned slab = gx * gy * {sz}u;
+      const unsigned zb = l / slab, rem = l - zb * slab;
+      const unsigned t = rem / {sz}u;
+      bz = zb * {sz}u + (rem - t * {sz}u);
+      bx = t % gx;
+      by = t / gx;
+    }}
+  }}
+"""
+
+
+def _prologue(interior: bool, xcd: bool = False, u32: bool = False, zs: int = 0) -> str:
     lo = "p.NG" if interior else "0"
     tail = U32_LINES if u32 else ""
+    if zs:
+        return _zsweep_remap(zs) + f"""  const int i0 = {lo} + (int)(bx * blockDim.x + threadIdx.x);
+  const int i1 = {lo} + (int)(by * blockDim.y + threadIdx.y);
+  const int i2 = {lo} + (int)bz;
+  if (i0 >= p.Nx0 - {lo} || i1 >= p.Nx1 - {lo}) return;
+  const long s1 = p.s1, s2 = p.s2, Ntot = p.Ntot;
+  const long idx = i0 + s1 * i1 + s2 * i2;
+""" + tail
     if xcd:
         return XCD_REMAP + f"""  const int i0 = {lo} + (int)(bx * blockDim.x + threadIdx.x);
   const int i1 = {lo} + (int)(by * blockDim.y + threadIdx.y);
@@ -759,10 +790,10 @@ def _group_bounds(lb: str, k: int, ngroups: int) -> str:
 # Candidate schedules per build: (kind, partition, launch bounds).
 FULL_CANDIDATES = {
     "rhs": ["p1_lb1", "p2_lb1_sj_ln_pf16_u32", "p2_lb1_sj_ln_pf16_u32_ck8", "p2_lb1_sj_ln_pf20_u32_ck10",
-            "p3_lb1_sj_ln_pf16_u32", "p4_lbm_sj_ln_pf8_u32", "p4_lbm_sj_ln_pf12_u32", "p4_lbm_sl_ln_pf8_u32",
-            "p5_lbm_sj_ln_pf8_u32", "p5_lbm_sj_ln_pf12_u32", "p5_lbm_sj_ln_u32", "p5_lbm_sj_ln_pf4_u32"],
-    "ricci": ["r1_lb1", "r2_lb1_sj_pf32", "r2_lb1_sj_pf20_u32_ck10", "r2_lb1_sj_pf16_u32_ck8", "r6_lb2_sj_u32",
-              "r6_lb2_sj_pf4_u32", "r6_lb2_sj_pf8_u32", "r6_lb2_sl_pf8_u32"],
+            "p3_lb1_sj_ln_pf16_u32", "p2_lb1_sj_ln_pf16_u32_ck8_zs8", "p2_lb1_sj_ln_pf16_u32_ck8_zs16",
+            "p2_lb1_sj_ln_pf16_u32_ck8_zs32", "p3_lb1_sj_ln_pf16_u32_zs16"],
+    "ricci": ["r1_lb1", "r2_lb1_sj_pf32", "r2_lb1_sj_pf20_u32_ck10", "r2_lb1_sj_pf16_u32_ck8", "r2_lb1_sj_pf32_zs8",
+              "r2_lb1_sj_pf32_zs16", "r2_lb1_sj_pf32_zs32", "r2_lb1_sj_pf16_u32_ck8_zs16"],
 }
 SMALL_CANDIDATES = {"rhs": ["p1_lb1", "p1_lb1_sl", "p3_lb1_sl", "p1_lb1_sl_ln_pf16", "p3_lb1_sl_ln_pf16",
                             "p1_lb1_sl_ln_pf16_u32", "p3_lb1_sl_ln_pf16_u32", "p3_lb1_sl_ln_pf16_u32_ck8"],
@@ -798,6 +829,7 @@ def candidate_tu(v: specs.Variant, fd_order: int, kind: str, cand: str, ensemble
     xcd = "xcd" in opts
     u32 = "u32" in opts
     chunk = next((int(o[2:]) for o in opts if o.startswith("ck") and o[2:].isdigit()), 0)
+    zs = next((int(o[2:]) for o in opts if o.startswith("zs") and o[2:].isdigit()), 0)
     rats: Dict[str, str] = {}
     kernels, calls, total = [], [], 0
     ens_kernels, ens_calls = [], []
@@ -828,7 +860,7 @@ def candidate_tu(v: specs.Variant, fd_order: int, kind: str, cand: str, ensemble
 extern "C" __global__ void __launch_bounds__({_group_bounds(lb, k, len(groups))})
 {kname}(const double *__restrict__ in, const double *__restrict__ aux, double *__restrict__ out,
         const double *__restrict__ rfm0, const double *__restrict__ rfm1, const bhah_kparams p) {{
-{_prologue(True, xcd, u32)}
+{_prologue(True, xcd, u32, zs)}
 {upw}
 {body}
 }}
