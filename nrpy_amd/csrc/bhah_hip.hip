@@ -103,6 +103,8 @@ struct bhah_patch {
   int *d_inner_dst = nullptr, *d_inner_src = nullptr;
   unsigned short *d_inner_mask = nullptr;  // bit t set: parity type t flips sign
   int16_t *d_outer[8][3] = {};
+  int *d_rad_lines = nullptr;  // (i1, i2) per radial line when the outer points form lines
+  int n_rad_lines = 0;
   double *d_wavespeed = nullptr, *d_finf = nullptr;
   double *d_red = nullptr;
   int *d_gf_parity = nullptr;
@@ -230,6 +232,26 @@ bc_radiation_kernel(const int16_t *__restrict__ pts, int npts, int ngfs, const d
   const int p = (int)(t % npts), gf = (int)(t / npts);
   rad_point(&pts[6 * p], gf, gfs, rhs, xx0, wavespeed[gf], finf[gf], st, R, coord, AMPL, SINHW, Nx0, s1, s2, Ntot,
             invdxx0);
+}
+
+// All ghost layers of the radiation condition in one launch, for grids whose pure outer
+// points are radial lines (i1, i2 fixed, i0 = Nx0-NG .. Nx0-1; spherical-like coordinates):
+// one thread per (line, gridfunction) walks the layers innermost first, so each layer reads
+// the previous layer's value it depends on from its own earlier store.
+__global__ void __launch_bounds__(256)
+bc_radiation_lines_kernel(const int *__restrict__ lines, int nlines, int ngfs, int NG, const double *__restrict__ gfs,
+                          double *__restrict__ rhs, const double *__restrict__ xx0,
+                          const double *__restrict__ wavespeed, const double *__restrict__ finf, RadStencil st,
+                          int R, int coord, double AMPL, double SINHW, int Nx0, long s1, long s2, long Ntot,
+                          double invdxx0) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)nlines * ngfs) return;
+  const int l = (int)(t % nlines), gf = (int)(t / nlines);
+  int16_t q[6] = {0, (int16_t)lines[2 * l], (int16_t)lines[2 * l + 1], -1, 0, 0};
+  for (int gz = 0; gz < NG; gz++) {
+    q[0] = (int16_t)(Nx0 - NG + gz);
+    rad_point(q, gf, gfs, rhs, xx0, wavespeed[gf], finf[gf], st, R, coord, AMPL, SINHW, Nx0, s1, s2, Ntot, invdxx0);
+  }
 }
 
 // Batched forms over an ensemble: one record per (member, boundary point), one
@@ -792,6 +814,30 @@ int bhah_hip_patch_create(const bhah_patch_desc *desc, bhah_patch **out) {
         bhah_hip_patch_destroy(P);
         return rc;
       }
+  // radial-line form of the outer points (every layer lists the same (i1, i2) on the x0max face)
+  if (d.coord != BHAH_CARTESIAN && P->NG > 0) {
+    bool lines = true;
+    const auto &L0 = P->outer[0][0];
+    for (int gz = 0; gz < P->NG && lines; gz++) {
+      if (!P->outer[gz][1].empty() || !P->outer[gz][2].empty() || P->outer[gz][0].size() != L0.size()) lines = false;
+      for (size_t q = 0; lines && q < L0.size(); q += 6) {
+        const int16_t *a = &P->outer[gz][0][q], *b = &L0[q];
+        if (a[0] != P->Nx[0] - P->NG + gz || a[1] != b[1] || a[2] != b[2] || a[3] != -1 || a[4] || a[5]) lines = false;
+      }
+    }
+    if (lines && !L0.empty()) {
+      std::vector<int> ln;
+      for (size_t q = 0; q < L0.size(); q += 6) {
+        ln.push_back(L0[q + 1]);
+        ln.push_back(L0[q + 2]);
+      }
+      if ((rc = upload(ln, &P->d_rad_lines))) {
+        bhah_hip_patch_destroy(P);
+        return rc;
+      }
+      P->n_rad_lines = (int)(L0.size() / 6);
+    }
+  }
   if (hipMalloc((void **)&P->d_red, 8 * sizeof(double)) != hipSuccess) {
     bhah_hip_patch_destroy(P);
     return fail(BHAH_ERR_HIP, "hipMalloc failed");
@@ -815,6 +861,7 @@ int bhah_hip_patch_destroy(bhah_patch *P) {
   hipFree(P->d_red);
   hipFree(P->d_gf_parity);
   hipFree(P->d_adj);
+  hipFree(P->d_rad_lines);
   for (int gz = 0; gz < 8; gz++)
     for (int d = 0; d < 3; d++) hipFree(P->d_outer[gz][d]);
   if (P->own_stream) hipStreamDestroy(P->stream);
@@ -952,6 +999,15 @@ int bhah_hip_apply_bcs_outerradiation_and_inner(bhah_patch *P, const double *gfs
   if (R < 1 || R > 3) return fail(BHAH_ERR_ARG, "radiation_bc_fd_order must be 2, 4 or 6");
   RadStencil st;
   rad_stencil(R, st);
+  if (P->n_rad_lines) {
+    dim3 g, b;
+    launch_1d((long)P->n_rad_lines * P->ngfs, g, b);
+    hipLaunchKernelGGL(bc_radiation_lines_kernel, g, b, 0, P->stream, P->d_rad_lines, P->n_rad_lines, P->ngfs, P->NG,
+                       gfs, rhs, P->d_xx0, P->d_wavespeed, P->d_finf, st, R, P->d.coord, P->d.grid_physical_size,
+                       P->d.SINHW, P->Nx[0], (long)P->Nx[0], (long)P->Nx[0] * P->Nx[1], P->Ntot, P->invdxx[0]);
+    HIP_TRY(hipGetLastError());
+    return bhah_hip_apply_bcs_inner_only(P, rhs);
+  }
   for (int gz = 0; gz < P->NG; gz++)
     for (int d = 0; d < 3; d++) {
       const int n = (int)(P->outer[gz][d].size() / 6);
