@@ -309,6 +309,11 @@ def _ordered_body(spec: specs.Spec, fd_order: int, out_lvalues: List[str], ratio
     def need(expr, indent="  "):
         for sy in sorted(expr.free_symbols, key=str):
             n = str(sy)
+            if n in defs and n not in emitted and linear == "fused" and n in derivs:
+                dv = derivs[n]
+                if dv.op in ("dD", "dDD") and len(set(dv.dirs)) == 1 and (dv.gf, dv.dirs[0]) in line_out \
+                        and (dv.gf, dv.dirs[0]) not in lines_done:
+                    emit_line((dv.gf, dv.dirs[0]))
             if n in defs and n not in emitted:
                 emitted.add(n)
                 if n in derivs:
@@ -361,57 +366,94 @@ def _ordered_body(spec: specs.Spec, fd_order: int, out_lvalues: List[str], ratio
     w_dn = dict(fd.weights_1d("ddnD", fd_order))
     w_ko = dict(fd.weights_1d("dKOD", fd_order))
 
-    def emit_lines(g):
-        """lin_<g> from shared line loads (linear == "lines")."""
-        acc = f"lin_{g}"
-        acc_names[g] = acc
-        code(f"  double {acc} = 0.0;")
-        per: Dict[Tuple[str, int], Dict[str, str]] = {}
-        for d, c in terms_sorted(lin_terms[g]):
-            per.setdefault((d.gf, d.dirs[0]), {})["up" if d.op == "dupD" else "ko"] = cname[c]
-        for (gf, dirn), cf in sorted(per.items()):
-            gid[1] += 1
-            if gf in defs:
-                need(sp.Symbol(gf, real=True))
-                f0 = gf
-            else:
-                f0 = f"ctr{gid[1]}_{gf}"
-                if split_loads:
-                    loads([(f0, st.load(gf, (0, 0, 0)))])
-                else:
-                    code(f"  const double {f0} = {st.load(gf, (0, 0, 0))};")
-            pts = {}
-            ld = []
-            for k in range(-R, R + 1):
-                if k == 0:
-                    continue
-                o = [0, 0, 0]
-                o[dirn] = k
-                name = f"ln{gid[1]}_{k + R}"
-                ld.append((name, st.load(gf, tuple(o))))
-                pts[k] = name
-            pts[0] = f0
+    w_dD = dict(fd.weights_1d("dD", fd_order))
+    w_dDD = dict(fd.weights_1d("dDD", fd_order))
+    line_cf: Dict[Tuple[str, int], Dict[str, str]] = {}
+    line_out: Dict[Tuple[str, int], str] = {}
+    for g_out, tl in lin_terms.items():
+        for d, c in terms_sorted(tl):
+            key = (d.gf, d.dirs[0])
+            line_cf.setdefault(key, {})["up" if d.op == "dupD" else "ko"] = cname[c]
+            line_out[key] = g_out
+    lines_done: Set[Tuple[str, int]] = set()
+
+    def emit_line(key):
+        """Load one (gridfunction, direction) line of 2R+1 points once; form the output's
+        upwind/KO contribution and, with linear == "fused", every centred derivative of that
+        gridfunction along that direction the kernel uses."""
+        lines_done.add(key)
+        gf, dirn = key
+        g = line_out[key]
+        cf = line_cf[key]
+        gid[1] += 1
+        if gf in defs:
+            need(sp.Symbol(gf, real=True))
+            f0 = gf
+        else:
+            f0 = f"ctr{gid[1]}_{gf}"
             if split_loads:
-                loads(ld)
+                loads([(f0, st.load(gf, (0, 0, 0)))])
             else:
-                for name, e in ld:
-                    code(f"  const double {name} = {e};")
-            terms_c = []
-            if "up" in cf:
-                # both one-sided sums, then a select (no divergent branch)
-                su = " + ".join(f"{_dbl(w)} * {pts[k]}" for k, w in sorted(w_up.items()))
-                sd = " + ".join(f"{_dbl(w)} * {pts[k]}" for k, w in sorted(w_dn.items()))
-                code(f"  const double su{gid[1]} = {su};")
-                code(f"  const double sd{gid[1]} = {sd};")
-                terms_c.append(f"{cf['up']} * (p.invdxx{dirn} * ((betaU{dirn} > 0.0) ? su{gid[1]} : sd{gid[1]}))")
-                flops[0] += 2 * (2 * len(w_up) - 1) + 2
-            if "ko" in cf:
-                sk = [f"{_dbl(w_ko[0])} * {pts[0]}"]
-                sk += [f"{_dbl(w_ko[k])} * ({pts[k]} + {pts[-k]})" for k in sorted(w_ko) if k > 0]
-                terms_c.append(f"{cf['ko']} * (p.invdxx{dirn} * ({' + '.join(sk)}))")
-                flops[0] += 3 * (len(w_ko) // 2) + 3
-            code(f"  {acc} += {' + '.join(terms_c)};")
-            barrier()
+                code(f"  const double {f0} = {st.load(gf, (0, 0, 0))};")
+        pts = {}
+        ld = []
+        for k in range(-R, R + 1):
+            if k == 0:
+                continue
+            o = [0, 0, 0]
+            o[dirn] = k
+            name = f"ln{gid[1]}_{k + R}"
+            ld.append((name, st.load(gf, tuple(o))))
+            pts[k] = name
+        pts[0] = f0
+        if split_loads:
+            loads(ld)
+        else:
+            for name, e in ld:
+                code(f"  const double {name} = {e};")
+        if linear == "fused":
+            for n, dv in sorted(derivs.items()):
+                if n in emitted or dv.gf != gf:
+                    continue
+                if dv.op == "dD" and dv.dirs[0] == dirn:
+                    parts = [f"{_dbl(w_dD[k])} * ({pts[k]} - {pts[-k]})" for k in sorted(w_dD) if k > 0]
+                    code(f"  const double {n} = p.invdxx{dirn} * ({' + '.join(parts)});")
+                elif dv.op == "dDD" and dv.dirs == (dirn, dirn):
+                    parts = [f"{_dbl(w_dDD[0])} * {pts[0]}"]
+                    parts += [f"{_dbl(w_dDD[k])} * ({pts[k]} + {pts[-k]})" for k in sorted(w_dDD) if k > 0]
+                    code(f"  const double {n} = p.invdxx{dirn} * p.invdxx{dirn} * ({' + '.join(parts)});")
+                else:
+                    continue
+                emitted.add(n)
+                flops[0] += stencil_flops(dv, fd_order)
+        acc = f"lin_{g}"
+        if g not in acc_names:
+            acc_names[g] = acc
+            code(f"  double {acc} = 0.0;")
+        terms_c = []
+        if "up" in cf:
+            # both one-sided sums, then a select (no divergent branch)
+            su = " + ".join(f"{_dbl(w)} * {pts[k]}" for k, w in sorted(w_up.items()))
+            sd = " + ".join(f"{_dbl(w)} * {pts[k]}" for k, w in sorted(w_dn.items()))
+            code(f"  const double su{gid[1]} = {su};")
+            code(f"  const double sd{gid[1]} = {sd};")
+            terms_c.append(f"{cf['up']} * (p.invdxx{dirn} * ((betaU{dirn} > 0.0) ? su{gid[1]} : sd{gid[1]}))")
+            flops[0] += 2 * (2 * len(w_up) - 1) + 2
+        if "ko" in cf:
+            sk = [f"{_dbl(w_ko[0])} * {pts[0]}"]
+            sk += [f"{_dbl(w_ko[k])} * ({pts[k]} + {pts[-k]})" for k in sorted(w_ko) if k > 0]
+            terms_c.append(f"{cf['ko']} * (p.invdxx{dirn} * ({' + '.join(sk)}))")
+            flops[0] += 3 * (len(w_ko) // 2) + 3
+        code(f"  {acc} += {' + '.join(terms_c)};")
+        barrier()
+
+    def emit_lines(g):
+        """All remaining lines of output g, just before it is stored."""
+        for key in sorted(k for k, o in line_out.items() if o == g and k not in lines_done):
+            emit_line(key)
+        if g not in acc_names:
+            acc_names[g] = f"lin_{g}"
+            code(f"  double lin_{g} = 0.0;")
 
     sdefs = spec.used_defs()
     if sdefs and cse_scope == "local":
@@ -789,14 +831,15 @@ def _group_bounds(lb: str, k: int, ngroups: int) -> str:
 
 # Candidate schedules per build: (kind, partition, launch bounds).
 FULL_CANDIDATES = {
-    "rhs": ["p1_lb1", "p2_lb1_sj_ln_pf16_u32", "p2_lb1_sj_ln_pf16_u32_ck8", "p2_lb1_sj_ln_pf20_u32_ck10",
-            "p3_lb1_sj_ln_pf16_u32", "p2_lb1_sj_ln_pf16_u32_ck8_zs8", "p2_lb1_sj_ln_pf16_u32_ck8_zs16",
-            "p2_lb1_sj_ln_pf16_u32_ck8_zs32", "p3_lb1_sj_ln_pf16_u32_zs16"],
-    "ricci": ["r1_lb1", "r2_lb1_sj_pf32", "r2_lb1_sj_pf20_u32_ck10", "r2_lb1_sj_pf16_u32_ck8", "r2_lb1_sj_pf32_zs8",
-              "r2_lb1_sj_pf32_zs16", "r2_lb1_sj_pf32_zs32", "r2_lb1_sj_pf16_u32_ck8_zs16"],
+    "rhs": ["p1_lb1", "p2_lb1_sj_ln_pf16_u32", "p2_lb1_sj_ln_pf16_u32_ck8", "p3_lb1_sj_ln_pf16_u32",
+            "p2_lb1_sj_lf_pf16_u32_ck8", "p2_lb1_sj_lf_pf12_u32_ck8", "p2_lb1_sj_lf_pf20_u32_ck10",
+            "p3_lb1_sj_lf_pf16_u32_ck8", "p2_lb1_sj_lf_pf16_u32_ck8_zs16", "p2_lb1_sl_lf_pf12_u32_ck8"],
+    "ricci": ["r1_lb1", "r2_lb1_sj_pf32", "r2_lb1_sj_pf20_u32_ck10", "r2_lb1_sj_pf32_zs8", "r2_lb1_sj_pf32_zs16",
+              "r2_lb1_sj_pf24_zs16", "r2_lb1_sj_pf32_ck16_zs16"],
 }
 SMALL_CANDIDATES = {"rhs": ["p1_lb1", "p1_lb1_sl", "p3_lb1_sl", "p1_lb1_sl_ln_pf16", "p3_lb1_sl_ln_pf16",
-                            "p1_lb1_sl_ln_pf16_u32", "p3_lb1_sl_ln_pf16_u32", "p3_lb1_sl_ln_pf16_u32_ck8"],
+                            "p3_lb1_sl_ln_pf16_u32", "p3_lb1_sl_ln_pf16_u32_ck8", "p3_lb1_sl_lf_pf16_u32_ck8",
+                            "p1_lb1_sl_lf_pf16_u32"],
                     "ricci": ["r1_lb1", "r1_lb1_sl", "r2_lb1_sl", "r1_lb1_pf16", "r2_lb1_sl_pf16", "r1_lb1_pf16_u32",
                               "r2_lb1_sl_pf16_u32", "r2_lb1_sl_pf16_u32_ck8"]}
 
@@ -823,7 +866,7 @@ def candidate_tu(v: specs.Variant, fd_order: int, kind: str, cand: str, ensemble
     mode = next((o for o in opts if o in ("x", "sj", "sl")), "x")
     staged = mode in ("sj", "sl")
     scope = "local" if mode == "sl" else "joint"
-    linear = "lines" if "ln" in opts else "stream"
+    linear = "fused" if "lf" in opts else ("lines" if "ln" in opts else "stream")
     prefetch = next((int(o[2:]) for o in opts if o.startswith("pf")), 0)
     ahead = next((int(o[1:]) for o in opts if o[0] == "a" and o[1:].isdigit()), 12)
     xcd = "xcd" in opts
