@@ -311,9 +311,13 @@ def _ordered_body(spec: specs.Spec, fd_order: int, out_lvalues: List[str], ratio
             n = str(sy)
             if n in defs and n not in emitted and linear == "fused" and n in derivs:
                 dv = derivs[n]
-                if dv.op in ("dD", "dDD") and len(set(dv.dirs)) == 1 and (dv.gf, dv.dirs[0]) in line_out \
-                        and (dv.gf, dv.dirs[0]) not in lines_done:
-                    emit_line((dv.gf, dv.dirs[0]))
+                if dv.op in ("dD", "dDD") and len(set(dv.dirs)) == 1:
+                    key = (dv.gf, dv.dirs[0])
+                    if key in line_out:
+                        if key not in lines_done:
+                            emit_line(key)
+                    elif key not in centred_done and dv.gf in spec.gfs:
+                        emit_centred_line(key)
             if n in defs and n not in emitted:
                 emitted.add(n)
                 if n in derivs:
@@ -446,6 +450,50 @@ def _ordered_body(spec: specs.Spec, fd_order: int, out_lvalues: List[str], ratio
             flops[0] += 3 * (len(w_ko) // 2) + 3
         code(f"  {acc} += {' + '.join(terms_c)};")
         barrier()
+
+    centred_done: Set[Tuple[str, int]] = set()
+
+    def emit_centred_line(key):
+        """Load the 2*(fd/2)+1 centred points of a (gridfunction, direction) line once and
+        form both its first and its unmixed second derivative where the kernel uses them."""
+        centred_done.add(key)
+        gf, dirn = key
+        want = [(n, dv) for n, dv in sorted(derivs.items()) if n not in emitted and dv.gf == gf and
+                ((dv.op == "dD" and dv.dirs[0] == dirn) or (dv.op == "dDD" and dv.dirs == (dirn, dirn)))]
+        if len(want) < 2:
+            return False
+        gid[1] += 1
+        pts = {0: gf}
+        if any(dv.op == "dDD" for _, dv in want):
+            if gf in defs:
+                need(sp.Symbol(gf, real=True))
+            else:
+                pts[0] = f"clc{gid[1]}"
+                code(f"  const double {pts[0]} = {st.load(gf, (0, 0, 0))};")
+        ld = []
+        for k in sorted(w_dD):
+            o = [0, 0, 0]
+            o[dirn] = k
+            name = f"cl{gid[1]}_{k + R}"
+            ld.append((name, st.load(gf, tuple(o))))
+            pts[k] = name
+        if split_loads:
+            loads(ld)
+        else:
+            for name, e in ld:
+                code(f"  const double {name} = {e};")
+        for n, dv in want:
+            if dv.op == "dD":
+                parts = [f"{_dbl(w_dD[k])} * ({pts[k]} - {pts[-k]})" for k in sorted(w_dD) if k > 0]
+                code(f"  const double {n} = p.invdxx{dirn} * ({' + '.join(parts)});")
+            else:
+                parts = [f"{_dbl(w_dDD[0])} * {pts[0]}"]
+                parts += [f"{_dbl(w_dDD[k])} * ({pts[k]} + {pts[-k]})" for k in sorted(w_dDD) if k > 0]
+                code(f"  const double {n} = p.invdxx{dirn} * p.invdxx{dirn} * ({' + '.join(parts)});")
+            emitted.add(n)
+            flops[0] += stencil_flops(dv, fd_order)
+        barrier()
+        return True
 
     def emit_lines(g):
         """All remaining lines of output g, just before it is stored."""
@@ -833,15 +881,15 @@ def _group_bounds(lb: str, k: int, ngroups: int) -> str:
 FULL_CANDIDATES = {
     "rhs": ["p1_lb1", "p2_lb1_sj_ln_pf16_u32", "p2_lb1_sj_ln_pf16_u32_ck8", "p3_lb1_sj_ln_pf16_u32",
             "p2_lb1_sj_lf_pf16_u32_ck8", "p2_lb1_sj_lf_pf12_u32_ck8", "p2_lb1_sj_lf_pf20_u32_ck10",
-            "p3_lb1_sj_lf_pf16_u32_ck8", "p2_lb1_sj_lf_pf16_u32_ck8_zs16", "p2_lb1_sl_lf_pf12_u32_ck8"],
-    "ricci": ["r1_lb1", "r2_lb1_sj_pf32", "r2_lb1_sj_pf20_u32_ck10", "r2_lb1_sj_pf32_zs8", "r2_lb1_sj_pf32_zs16",
-              "r2_lb1_sj_pf24_zs16", "r2_lb1_sj_pf32_ck16_zs16"],
+            "p3_lb1_sj_lf_pf16_u32_ck8", "p2_lb1_sj_lf_pf12_u32_ck8_zs16", "p2_lb1_sl_lf_pf12_u32_ck8"],
+    "ricci": ["r1_lb1", "r2_lb1_sj_pf32", "r2_lb1_sj_pf24_zs16", "r2_lb1_sj_lf_pf24_zs16", "r2_lb1_sj_lf_pf16_zs16",
+              "r2_lb1_sj_lf_pf32", "r2_lb1_sj_lf_pf24_u32_zs16", "r3_lb1_sj_lf_pf24_zs16"],
 }
 SMALL_CANDIDATES = {"rhs": ["p1_lb1", "p1_lb1_sl", "p3_lb1_sl", "p1_lb1_sl_ln_pf16", "p3_lb1_sl_ln_pf16",
                             "p3_lb1_sl_ln_pf16_u32", "p3_lb1_sl_ln_pf16_u32_ck8", "p3_lb1_sl_lf_pf16_u32_ck8",
                             "p1_lb1_sl_lf_pf16_u32"],
                     "ricci": ["r1_lb1", "r1_lb1_sl", "r2_lb1_sl", "r1_lb1_pf16", "r2_lb1_sl_pf16", "r1_lb1_pf16_u32",
-                              "r2_lb1_sl_pf16_u32", "r2_lb1_sl_pf16_u32_ck8"]}
+                              "r2_lb1_sl_pf16_u32", "r2_lb1_sl_pf16_u32_ck8", "r2_lb1_sl_lf_pf16_u32"]}
 
 CAND_SIG = ("dim3 g, dim3 b, hipStream_t s, const double *in, const double *aux, double *out, "
             "const double *rfm0, const double *rfm1, bhah_kparams p")
@@ -896,9 +944,10 @@ def candidate_tu(v: specs.Variant, fd_order: int, kind: str, cand: str, ensemble
         else:
             lv = [f"out[{bssn.AUXEVOL_GFS.index(g)} * Ntot + idx]" for g, _ in sub.outputs]
             body = _ordered_body(sub, fd_order, lv, rats, cse_scope=scope, prefetch=prefetch, ahead=ahead, u32=u32,
-                                 chunk=chunk)
+                                 chunk=chunk, linear="fused" if linear == "fused" else "stream")
             upw = ""
-        total += FLOPS[f"{sub.name}_fd{fd_order}{_mode_suffix(linear if kind == 'rhs' else 'stream', prefetch)}"]
+        used_linear = linear if kind == "rhs" else ("fused" if linear == "fused" else "stream")
+        total += FLOPS[f"{sub.name}_fd{fd_order}{_mode_suffix(used_linear, prefetch)}"]
         kernels.append(f"""
 extern "C" __global__ void __launch_bounds__({_group_bounds(lb, k, len(groups))})
 {kname}(const double *__restrict__ in, const double *__restrict__ aux, double *__restrict__ out,
