@@ -68,6 +68,7 @@ class StencilEmitter:
         # per-lane address arithmetic or address registers.  PAD keeps every uniform
         # offset non-negative.
         self.u32 = u32
+        self.info: Dict[str, tuple] = {}  # buffer-load expression -> (array, gridfunction, offset)
         self.R = fd.ghost_zones(fd_order)
         self._collect: List[Tuple[str, str]] | None = None
         self._nload = 0
@@ -100,7 +101,9 @@ class StencilEmitter:
                 t.append(f"{off[1]} * s1")
             if off[2]:
                 t.append(f"{off[2]} * s2")
-            return f"BLD(rsrc_{self.array}, {' + '.join(t)})"
+            e = f"BLD(rsrc_{self.array}, {' + '.join(t)})"
+            self.info[e] = (self.array, g, tuple(off))
+            return e
         terms = []
         if off[0]:
             terms.append(f"{off[0]}")
@@ -264,7 +267,7 @@ def _emit_defs(spec: specs.Spec, fd_order: int, u32: bool = False):
 def _ordered_body(spec: specs.Spec, fd_order: int, out_lvalues: List[str], rationals: Dict[str, str],
                   stream_linear: bool = True, order: str = "dfs", barriers: bool = True,
                   cse_scope: str = "joint", linear: str = "stream", prefetch: int = 0,
-                  ahead: int = 12, u32: bool = False, chunk: int = 0) -> str:
+                  ahead: int = 12, u32: bool = False, chunk: int = 0, wide: bool = False) -> str:
     """Stencils and CSE'd algebra for one pointwise spec.
 
     With ``stream_linear``, upwind and KO derivatives are consumed as soon as
@@ -293,6 +296,7 @@ def _ordered_body(spec: specs.Spec, fd_order: int, out_lvalues: List[str], ratio
     flops = [0]
     gid = [0, 0]  # load-group counter, line counter
     split_loads = prefetch > 0
+    wide = wide and u32 and split_loads
 
     def code(t):
         items.append(("C", t))
@@ -303,6 +307,35 @@ def _ordered_body(spec: specs.Spec, fd_order: int, out_lvalues: List[str], ratio
 
     def loads(pairs):
         gid[0] += 1
+        if wide:
+            # two values adjacent along xx0 (same array, gridfunction, xx1/xx2 offset) come
+            # from one 16-byte buffer load
+            info = {name: st.info.get(e) for name, e in pairs}
+            by_key = {}
+            for name, e in pairs:
+                if info[name]:
+                    a, g, off = info[name]
+                    by_key[(a, g, off[1], off[2], off[0])] = name
+            used = set()
+            for name, e in pairs:
+                if name in used:
+                    continue
+                inf = info[name]
+                if inf:
+                    a, g, off = inf
+                    mate = by_key.get((a, g, off[1], off[2], off[0] + 1))
+                    if mate and mate not in used:
+                        used |= {name, mate}
+                        so = e[e.index(",") + 1:-1].strip()
+                        pr = f"w{name}"
+                        items.append(("L", f"  const auto {pr} = __builtin_amdgcn_raw_buffer_load_b128(rsrc_{a}, vo, "
+                                           f"(unsigned)(({so}) * 8), 0);", gid[0]))
+                        items.append(("L", f"  const double {name} = BHAH_LO({pr});", gid[0]))
+                        items.append(("L", f"  const double {mate} = BHAH_HI({pr});", gid[0]))
+                        continue
+                used.add(name)
+                items.append(("L", f"  const double {name} = {e};", gid[0]))
+            return
         for name, e in pairs:
             items.append(("L", f"  const double {name} = {e};", gid[0]))
 
@@ -680,6 +713,8 @@ U32_LINES = """  const long PAD = (long)p.NG * (1 + s1 + s2);
 """
 # Raw buffer load of one double: per-lane byte offset vo, uniform element offset so.
 BLD_MACRO = """#define BLD(r, so) __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64((r), vo, (unsigned)((so) * 8), 0))
+#define BHAH_LO(v) __builtin_bit_cast(double, ((unsigned long long)(v)[1] << 32) | (unsigned long long)(v)[0])
+#define BHAH_HI(v) __builtin_bit_cast(double, ((unsigned long long)(v)[3] << 32) | (unsigned long long)(v)[2])
 """
 
 
@@ -879,17 +914,18 @@ def _group_bounds(lb: str, k: int, ngroups: int) -> str:
 
 # Candidate schedules per build: (kind, partition, launch bounds).
 FULL_CANDIDATES = {
-    "rhs": ["p1_lb1", "p2_lb1_sj_ln_pf16_u32", "p2_lb1_sj_ln_pf16_u32_ck8", "p3_lb1_sj_ln_pf16_u32",
-            "p2_lb1_sj_lf_pf16_u32_ck8", "p2_lb1_sj_lf_pf12_u32_ck8", "p2_lb1_sj_lf_pf20_u32_ck10",
-            "p3_lb1_sj_lf_pf16_u32_ck8", "p2_lb1_sj_lf_pf12_u32_ck8_zs16", "p2_lb1_sl_lf_pf12_u32_ck8"],
-    "ricci": ["r1_lb1", "r2_lb1_sj_pf32", "r2_lb1_sj_pf24_zs16", "r2_lb1_sj_lf_pf24_zs16", "r2_lb1_sj_lf_pf16_zs16",
-              "r2_lb1_sj_lf_pf32", "r2_lb1_sj_lf_pf24_u32_zs16", "r3_lb1_sj_lf_pf24_zs16"],
+    "rhs": ["p1_lb1", "p2_lb1_sj_ln_pf16_u32_ck8", "p2_lb1_sj_lf_pf16_u32_ck8", "p2_lb1_sj_lf_pf12_u32_ck8",
+            "p2_lb1_sj_lf_pf12_u32_ck8_zs16", "p2_lb1_sj_lf_pf12_u32_ck8_w2", "p2_lb1_sj_lf_pf16_u32_ck8_w2",
+            "p2_lb1_sj_lf_pf8_u32_ck8_w2", "p2_lb1_sj_lf_pf12_u32_ck8_w2_zs16", "p3_lb1_sj_lf_pf12_u32_ck8_w2"],
+    "ricci": ["r1_lb1", "r2_lb1_sj_pf32", "r2_lb1_sj_lf_pf24_zs16", "r2_lb1_sj_lf_pf16_zs16", "r2_lb1_sj_lf_pf32",
+              "r2_lb1_sj_lf_pf16_u32_w2_zs16", "r2_lb1_sj_lf_pf24_u32_w2_zs16", "r2_lb1_sj_lf_pf16_u32_w2"],
 }
 SMALL_CANDIDATES = {"rhs": ["p1_lb1", "p1_lb1_sl", "p3_lb1_sl", "p1_lb1_sl_ln_pf16", "p3_lb1_sl_ln_pf16",
                             "p3_lb1_sl_ln_pf16_u32", "p3_lb1_sl_ln_pf16_u32_ck8", "p3_lb1_sl_lf_pf16_u32_ck8",
-                            "p1_lb1_sl_lf_pf16_u32"],
+                            "p1_lb1_sl_lf_pf16_u32", "p1_lb1_sl_lf_pf16_u32_w2"],
                     "ricci": ["r1_lb1", "r1_lb1_sl", "r2_lb1_sl", "r1_lb1_pf16", "r2_lb1_sl_pf16", "r1_lb1_pf16_u32",
-                              "r2_lb1_sl_pf16_u32", "r2_lb1_sl_pf16_u32_ck8", "r2_lb1_sl_lf_pf16_u32"]}
+                              "r2_lb1_sl_pf16_u32", "r2_lb1_sl_pf16_u32_ck8", "r2_lb1_sl_lf_pf16_u32",
+                              "r2_lb1_sl_lf_pf16_u32_w2"]}
 
 CAND_SIG = ("dim3 g, dim3 b, hipStream_t s, const double *in, const double *aux, double *out, "
             "const double *rfm0, const double *rfm1, bhah_kparams p")
@@ -921,6 +957,7 @@ def candidate_tu(v: specs.Variant, fd_order: int, kind: str, cand: str, ensemble
     u32 = "u32" in opts
     chunk = next((int(o[2:]) for o in opts if o.startswith("ck") and o[2:].isdigit()), 0)
     zs = next((int(o[2:]) for o in opts if o.startswith("zs") and o[2:].isdigit()), 0)
+    wide = "w2" in opts
     rats: Dict[str, str] = {}
     kernels, calls, total = [], [], 0
     ens_kernels, ens_calls = [], []
@@ -938,13 +975,13 @@ def candidate_tu(v: specs.Variant, fd_order: int, kind: str, cand: str, ensemble
         if kind == "rhs":
             lv = [f"out[{bssn.EVOL_GFS.index(g)} * Ntot + idx]" for g, _ in sub.outputs]
             body = _ordered_body(sub, fd_order, lv, rats, cse_scope=scope, linear=linear, prefetch=prefetch,
-                                 ahead=ahead, u32=u32, chunk=chunk)
+                                 ahead=ahead, u32=u32, chunk=chunk, wide=wide)
             has_up = any(d.op == "dupD" for d in sub.classify()[2])
             upw = _upwind_prologue(sub, fd_order, rats, tables=(linear == "stream"), u32=u32) if has_up else ""
         else:
             lv = [f"out[{bssn.AUXEVOL_GFS.index(g)} * Ntot + idx]" for g, _ in sub.outputs]
             body = _ordered_body(sub, fd_order, lv, rats, cse_scope=scope, prefetch=prefetch, ahead=ahead, u32=u32,
-                                 chunk=chunk, linear="fused" if linear == "fused" else "stream")
+                                 chunk=chunk, linear="fused" if linear == "fused" else "stream", wide=wide)
             upw = ""
         used_linear = linear if kind == "rhs" else ("fused" if linear == "fused" else "stream")
         total += FLOPS[f"{sub.name}_fd{fd_order}{_mode_suffix(used_linear, prefetch)}"]
@@ -966,7 +1003,7 @@ extern "C" __global__ void __launch_bounds__({_group_bounds(lb, k, len(groups))}
     undefs = "\n".join(f"#undef {k}" for k in sorted(rats))
     if u32:
         consts = BLD_MACRO + consts
-        undefs += "\n#undef BLD"
+        undefs += "\n#undef BLD\n#undef BHAH_LO\n#undef BHAH_HI"
     launcher = f"""
 extern "C" hipError_t launch_bssn_{kind}_{tag}_{cand}({CAND_SIG}) {{
 {chr(10).join(calls)}

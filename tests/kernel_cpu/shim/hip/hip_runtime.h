@@ -38,6 +38,18 @@ inline double __builtin_amdgcn_raw_buffer_load_b64(__amdgpu_buffer_rsrc_t r, uns
   return v;
 }
 
+struct bhah_u4 {
+  unsigned v[4];
+  unsigned operator[](int i) const { return v[i]; }
+};
+inline bhah_u4 __builtin_amdgcn_raw_buffer_load_b128(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so, int) {
+  const unsigned long off = (unsigned long)vo + so;
+  if (r.base == nullptr || off + 16 > r.bytes) __builtin_trap();
+  bhah_u4 v;
+  __builtin_memcpy(&v, r.base + off, 16);
+  return v;
+}
+
 template <typename K, typename... A>
 void cpu_launch(K kernel, dim3 g, dim3 b, A... args) {
   gridDim = g;
