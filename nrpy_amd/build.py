@@ -12,6 +12,7 @@ from __future__ import annotations
 
 import concurrent.futures as cf
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -50,7 +51,7 @@ def _compile(src: str) -> str:
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr[-4000:]}")
-    if "_pf" in os.path.basename(src):
+    if re.search(r"_pf|_L\d", os.path.basename(src)):
         # early-issued stencil loads: verify the vmcnt range on the emitted assembly
         asm = obj[:-2] + ".s"
         r = subprocess.run([HIPCC, *HIPFLAGS, "--cuda-device-only", "-S", src, "-o", asm], capture_output=True,

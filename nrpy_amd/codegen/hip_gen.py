@@ -267,7 +267,8 @@ def _emit_defs(spec: specs.Spec, fd_order: int, u32: bool = False):
 def _ordered_body(spec: specs.Spec, fd_order: int, out_lvalues: List[str], rationals: Dict[str, str],
                   stream_linear: bool = True, order: str = "dfs", barriers: bool = True,
                   cse_scope: str = "joint", linear: str = "stream", prefetch: int = 0,
-                  ahead: int = 12, u32: bool = False, chunk: int = 0, wide: bool = False) -> str:
+                  ahead: int = 12, u32: bool = False, chunk: int = 0, wide: bool = False,
+                  defs_override: Dict[str, str] | None = None) -> str:
     """Stencils and CSE'd algebra for one pointwise spec.
 
     With ``stream_linear``, upwind and KO derivatives are consumed as soon as
@@ -285,8 +286,13 @@ def _ordered_body(spec: specs.Spec, fd_order: int, out_lvalues: List[str], ratio
     before the region that combines them, in consumption order, keeping at
     most B early-issued values in flight, so that load latency overlaps the
     arithmetic of earlier regions instead of stalling the single wave per SIMD.
+
+    ``defs_override`` maps symbol names to C expressions that replace their
+    loads or stencils (a caller that has already formed the derivatives).
     """
     defs = _emit_defs(spec, fd_order, u32)
+    if defs_override:
+        defs.update(defs_override)
     derivs = {d.symbol: d for d in spec.classify()[2]}
     centre_names = set(spec.classify()[0]) | set(spec.classify()[1])
     pr = emit.DoublePrinter(rationals)
@@ -914,18 +920,18 @@ def _group_bounds(lb: str, k: int, ngroups: int) -> str:
 
 # Candidate schedules per build: (kind, partition, launch bounds).
 FULL_CANDIDATES = {
-    "rhs": ["p1_lb1", "p2_lb1_sj_ln_pf16_u32_ck8", "p2_lb1_sj_lf_pf16_u32_ck8", "p2_lb1_sj_lf_pf12_u32_ck8",
-            "p2_lb1_sj_lf_pf12_u32_ck8_zs16", "p2_lb1_sj_lf_pf12_u32_ck8_w2", "p2_lb1_sj_lf_pf16_u32_ck8_w2",
-            "p2_lb1_sj_lf_pf8_u32_ck8_w2", "p2_lb1_sj_lf_pf12_u32_ck8_w2_zs16", "p3_lb1_sj_lf_pf12_u32_ck8_w2"],
-    "ricci": ["r1_lb1", "r2_lb1_sj_pf32", "r2_lb1_sj_lf_pf24_zs16", "r2_lb1_sj_lf_pf16_zs16", "r2_lb1_sj_lf_pf32",
-              "r2_lb1_sj_lf_pf16_u32_w2_zs16", "r2_lb1_sj_lf_pf24_u32_w2_zs16", "r2_lb1_sj_lf_pf16_u32_w2"],
+    "rhs": ["p1_lb1", "p2_lb1_sj_lf_pf12_u32_ck8", "p2_lb1_sj_lf_pf12_u32_ck8_w2", "p2_lb1_sj_lf_pf16_u32_ck8_w2",
+            "p2_lb1_sj_lf_pf16_u32_ck6_w2", "p2_lb1_sj_lf_pf16_u32_w2"],
+    "ricci": ["r1_lb1", "r2_lb1_sj_pf32", "r2_lb1_sj_lf_pf16_zs16", "r2_lb1_sj_lf_pf32", "r2_lb1_sj_lf_pf12_zs16",
+              "r2_lb1_sj_lf_pf16_u32_w2", "r1_lb1_sj_L16x4x4x1", "r2_lb1_sj_L16x4x4x1", "r2_lb1_sj_L32x4x4x2",
+              "r3_lb2_sj_L16x4x4x1"],
 }
 SMALL_CANDIDATES = {"rhs": ["p1_lb1", "p1_lb1_sl", "p3_lb1_sl", "p1_lb1_sl_ln_pf16", "p3_lb1_sl_ln_pf16",
                             "p3_lb1_sl_ln_pf16_u32", "p3_lb1_sl_ln_pf16_u32_ck8", "p3_lb1_sl_lf_pf16_u32_ck8",
                             "p1_lb1_sl_lf_pf16_u32", "p1_lb1_sl_lf_pf16_u32_w2"],
                     "ricci": ["r1_lb1", "r1_lb1_sl", "r2_lb1_sl", "r1_lb1_pf16", "r2_lb1_sl_pf16", "r1_lb1_pf16_u32",
                               "r2_lb1_sl_pf16_u32", "r2_lb1_sl_pf16_u32_ck8", "r2_lb1_sl_lf_pf16_u32",
-                              "r2_lb1_sl_lf_pf16_u32_w2"]}
+                              "r2_lb1_sl_lf_pf16_u32_w2", "r2_lb1_sj_L16x4x4x1"]}
 
 CAND_SIG = ("dim3 g, dim3 b, hipStream_t s, const double *in, const double *aux, double *out, "
             "const double *rfm0, const double *rfm1, bhah_kparams p")
@@ -934,6 +940,149 @@ CAND_SIG = ("dim3 g, dim3 b, hipStream_t s, const double *in, const double *aux,
 def _subspec(spec: specs.Spec, name: str, gfs: List[str]) -> specs.Spec:
     return specs.Spec(name, [(g, e) for g, e in spec.outputs if g in gfs], spec.gfs, spec.aux, spec.upwind,
                       defs=spec.defs)
+
+
+# An empty register-constrained asm: the value must exist in a VGPR at this point.
+LDS_PIN_MACRO = """#ifndef BHAH_PIN
+#define BHAH_PIN(x) asm volatile("" : "+v"(x))
+#endif
+"""
+
+
+class _LdsEmitter(StencilEmitter):
+    """Stencil expressions reading a workgroup's LDS box ``sh`` around box index ``c``."""
+
+    def __init__(self, spec: specs.Spec, fd_order: int, centre: str, BX: int, BY: int):
+        super().__init__(spec, fd_order)
+        self.centre, self.BX, self.BY = centre, BX, BY
+
+    def load(self, gf: str, off: Tuple[int, int, int]) -> str:
+        o = off[0] + self.BX * off[1] + self.BX * self.BY * off[2]
+        return f"sh[{self.centre}{f' + {o}' if o > 0 else (f' - {-o}' if o < 0 else '')}]"
+
+
+def is_lds(cand: str) -> bool:
+    """Whether a schedule name selects the workgroup-tiled (LDS) kernels."""
+    return any(o[:1] == "L" and o[1:2].isdigit() for o in cand.split("_"))
+
+
+def _lds_geometry(tok: str) -> Tuple[int, int, int, int]:
+    """``L<TX>x<TY>x<TZ>x<P>``: a TX x TY x TZ tile of interior points per workgroup,
+    P points per thread (spaced TZ/P apart along xx2)."""
+    TX, TY, TZ, P = (int(x) for x in tok[1:].split("x"))
+    if TZ % P or (TX * TY * TZ) % P:
+        raise ValueError(f"bad LDS tile {tok}")
+    return TX, TY, TZ, P
+
+
+def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str], geom: Tuple[int, int, int, int],
+               min_waves: int, scope: str, lv: List[str]) -> Tuple[str, str, int]:
+    """One workgroup-tiled kernel: each gridfunction with derivatives is staged, one at a
+    time, from global memory into a halo box in LDS (the next one's global loads in flight
+    while the current one's stencils read LDS), every derivative of the thread's points is
+    formed from the box, then the point algebra runs with all inputs in registers.
+
+    Stencil sums are the same expressions, in the same order, as the global-load kernels.
+    Returns (kernel source, launch statement, flops per point)."""
+    TX, TY, TZ, P = geom
+    NT = TX * TY * TZ // P
+    ZS = TZ // P
+    centre, auxn, derivs, rfm, params = sub.classify()
+    used: Set[str] = set()
+    for _, e in sub.outputs:
+        used |= {str(x) for x in e.free_symbols}
+    for _, e in sub.used_defs():
+        used |= {str(x) for x in e.free_symbols}
+    dused = [d for d in derivs if d.symbol in used]
+    if any(d.op in ("dupD", "dKOD") for d in dused):
+        raise ValueError("LDS kernels take centred derivatives only")
+    H = fd_order // 2
+    staged = [g for g in sub.gfs if any(d.gf == g for d in dused)]
+    BX, BY, BZ = TX + 2 * H, TY + 2 * H, TZ + 2 * H
+    NB = BX * BY * BZ
+    NL = (NB + NT - 1) // NT
+    # algebra for a placeholder point; each point substitutes its own register names
+    ovr: Dict[str, str] = {}
+    for n in centre:
+        if n in used:
+            ovr[n] = f"VPT_{n}" if n in staged else f"in[{sub.gfs.index(n)} * Ntot + idx]"
+    for d in dused:
+        ovr[d.symbol] = f"VPT_{d.symbol}"
+    body = _ordered_body(sub, fd_order, lv, rats, cse_scope=scope, linear="stream", prefetch=0, barriers=False,
+                         defs_override=ovr)
+    flops = FLOPS[f"{sub.name}_fd{fd_order}{_mode_suffix('stream', 0)}"]
+    L: List[str] = []
+    a = L.append
+    a(f"""
+extern "C" __global__ void __launch_bounds__({NT}{f', {min_waves}' if min_waves > 1 else ''})
+{kname}(const double *__restrict__ in, const double *__restrict__ aux, double *__restrict__ out,
+        const double *__restrict__ rfm0, const double *__restrict__ rfm1, const bhah_kparams p) {{
+  // Tile of {TX}x{TY}x{TZ} interior points; halo box {BX}x{BY}x{BZ} = {NB} values per gridfunction.
+  constexpr int BX = {BX}, BY = {BY}, NB = {NB}, NT = {NT}, NL = {NL};
+  __shared__ double sh[NB];
+  const int tid = (int)threadIdx.x;
+  const int lx = tid % {TX}, ly = (tid / {TX}) % {TY}, lz = tid / {TX * TY};
+  const long s1 = p.s1, s2 = p.s2, Ntot = p.Ntot;
+  const int o0 = p.NG + (int)blockIdx.x * {TX}, o1 = p.NG + (int)blockIdx.y * {TY}, o2 = p.NG + (int)blockIdx.z * {TZ};
+  // Box element e of this thread's j-th slot; indices are clamped into the array, so
+  // partial tiles at the upper interior edge read valid (unused) points.
+  int ofs[NL];
+#pragma unroll
+  for (int j = 0; j < NL; j++) {{
+    const int e = tid + NT * j;
+    const int bx = e % BX, by = (e / BX) % BY, bz = e / (BX * BY);
+    const int g0 = min(o0 - {H} + bx, p.Nx0 - 1), g1 = min(o1 - {H} + by, p.Nx1 - 1), g2 = min(o2 - {H} + bz, p.Nx2 - 1);
+    ofs[j] = g0 + (int)s1 * g1 + (int)s2 * g2;
+  }}""")
+    for k in range(P):
+        a(f"  const int c{k} = (lx + {H}) + BX * (ly + {H}) + BX * BY * (lz + {k * ZS + H});")
+    a("  double pf[NL];")
+
+    def fetch(g):
+        gi = sub.gfs.index(g)
+        a(f"""  {{
+    const double *src = in + {gi} * Ntot;
+#pragma unroll
+    for (int j = 0; j < NL; j++) pf[j] = src[ofs[j]];
+  }}""")
+
+    fetch(staged[0])
+    for si, g in enumerate(staged):
+        a(f"""  // ---- {g}
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < NL; j++) {{
+    const int e = tid + NT * j;
+    if (e < NB) sh[e] = pf[j];
+  }}
+  __syncthreads();""")
+        if si + 1 < len(staged):
+            fetch(staged[si + 1])
+        for k in range(P):
+            st = _LdsEmitter(sub, fd_order, f"c{k}", BX, BY)
+            # each value is pinned to a register where it is formed: otherwise the
+            # compiler sinks the stencil sums into the guarded algebra below and keeps
+            # every raw LDS value alive across the staging barriers instead
+            if g in ovr and g in used:
+                a(f"  double V{k}_{g} = sh[c{k}];")
+                a(f"  BHAH_PIN(V{k}_{g});")
+            for d in dused:
+                if d.gf == g:
+                    a(f"  double V{k}_{d.symbol} = {st.expr(d)};")
+                    a(f"  BHAH_PIN(V{k}_{d.symbol});")
+    for k in range(P):
+        a(f"""  {{
+    const int i0 = o0 + lx, i1 = o1 + ly, i2 = o2 + lz + {k * ZS};
+    if (i0 < p.Nx0 - p.NG && i1 < p.Nx1 - p.NG && i2 < p.Nx2 - p.NG) {{
+      const long idx = i0 + s1 * i1 + s2 * i2;
+{body.replace("VPT_", f"V{k}_")}
+    }}
+  }}""")
+    a("}")
+    launch = (f"  hipLaunchKernelGGL({kname}, dim3((p.Nx0 - 2 * p.NG + {TX - 1}) / {TX}, "
+              f"(p.Nx1 - 2 * p.NG + {TY - 1}) / {TY}, (p.Nx2 - 2 * p.NG + {TZ - 1}) / {TZ}), dim3({NT}), 0, s, "
+              f"in, aux, out, rfm0, rfm1, p);")
+    return "\n".join(L) + "\n", launch, flops
 
 
 def candidate_tu(v: specs.Variant, fd_order: int, kind: str, cand: str, ensemble: bool = False) -> Tuple[str, int]:
@@ -958,6 +1107,9 @@ def candidate_tu(v: specs.Variant, fd_order: int, kind: str, cand: str, ensemble
     chunk = next((int(o[2:]) for o in opts if o.startswith("ck") and o[2:].isdigit()), 0)
     zs = next((int(o[2:]) for o in opts if o.startswith("zs") and o[2:].isdigit()), 0)
     wide = "w2" in opts
+    lds = next((_lds_geometry(o) for o in opts if o[0] == "L" and o[1:2].isdigit()), None)
+    if lds and ensemble:
+        raise ValueError("LDS-tiled schedules have no batched form")
     rats: Dict[str, str] = {}
     kernels, calls, total = [], [], 0
     ens_kernels, ens_calls = [], []
@@ -972,6 +1124,14 @@ def candidate_tu(v: specs.Variant, fd_order: int, kind: str, cand: str, ensemble
         if scope == "local":
             sub = _subspec(sub, f"{sub.name}_local", [g for g, _ in sub.outputs])
         kname = f"bssn_{kind}_{tag}_{cand}_g{k}"
+        if lds:
+            outs = list(bssn.EVOL_GFS if kind == "rhs" else bssn.AUXEVOL_GFS)
+            src, launch, fl = lds_kernel(sub, fd_order, kname, rats, lds, 2 if lb == "lb2" else 1, scope,
+                                         [f"out[{outs.index(g)} * Ntot + idx]" for g, _ in sub.outputs])
+            kernels.append(src)
+            calls.append(launch)
+            total += fl
+            continue
         if kind == "rhs":
             lv = [f"out[{bssn.EVOL_GFS.index(g)} * Ntot + idx]" for g, _ in sub.outputs]
             body = _ordered_body(sub, fd_order, lv, rats, cse_scope=scope, linear=linear, prefetch=prefetch,
@@ -1001,6 +1161,8 @@ extern "C" __global__ void __launch_bounds__({_group_bounds(lb, k, len(groups))}
             ens_calls.append(f"  hipLaunchKernelGGL({ename}, g, b, 0, s, mem, items, nitems, in_sel, aux_sel, out_sel);")
     consts = "\n".join(f"#define {k} ({v})" for k, v in sorted(rats.items()))
     undefs = "\n".join(f"#undef {k}" for k in sorted(rats))
+    if lds:
+        consts = LDS_PIN_MACRO + consts
     if u32:
         consts = BLD_MACRO + consts
         undefs += "\n#undef BLD\n#undef BHAH_LO\n#undef BHAH_HI"
@@ -1216,10 +1378,11 @@ def generate(outdir: str) -> List[str]:
         with open(path, "w", encoding="utf-8") as f:
             f.write(hdr + bssn_kernels(v, order, ensemble=ens))
         paths.append(path)
-        arrays = {}
+        arrays, ncand = {}, {}
         for kind in ("rhs", "ricci"):
             rows = []
-            for cand in cands[kind]:
+            kcands = [c for c in cands[kind] if not (ens and is_lds(c))]
+            for cand in kcands:
                 src, fl = candidate_tu(v, order, kind, cand, ensemble=ens)
                 path = os.path.join(outdir, f"bssn_{tag}_{kind}_{cand}.hip")
                 with open(path, "w", encoding="utf-8") as f:
@@ -1233,11 +1396,12 @@ def generate(outdir: str) -> List[str]:
                 else:
                     rows.append(f'{{"{cand}", launch_bssn_{kind}_{tag}_{cand}, nullptr}}')
             arrays[kind] = f"kCand_{kind}_{tag}"
+            ncand[kind] = len(kcands)
             table_decl.append(f"static const BssnCandidate {arrays[kind]}[] = {{{', '.join(rows)}}};")
         sym_mask = sum(1 << int(c) for c in v.symmetry_axes)
         table_rows.append(
             f"    {{{COORD_ENUM[v.coord]}, {sym_mask}, {int(v.enable_KO)}, {order}, \"{tag}\", "
-            f"{arrays['rhs']}, {len(cands['rhs'])}, {arrays['ricci']}, {len(cands['ricci'])}, "
+            f"{arrays['rhs']}, {ncand['rhs']}, {arrays['ricci']}, {ncand['ricci']}, "
             f"launch_bssn_rk_det_{tag}, launch_bssn_detgamma_{tag}, launch_bssn_constraints_{tag}, "
             f"{f'launch_bssn_ens_rk_det_{tag}' if ens else 'nullptr'}, "
             + (f"adj_nslots_{tag}, launch_adj_rhs_{tag}, launch_adj_ricci_{tag}, launch_adj_det_{tag}, "

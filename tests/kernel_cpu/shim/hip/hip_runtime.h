@@ -13,7 +13,11 @@ typedef int hipError_t;
 typedef void *hipStream_t;
 #define __constant__
 static const hipError_t hipSuccess = 0;
-inline dim3 blockIdx, threadIdx, blockDim, gridDim;
+inline dim3 blockIdx, blockDim, gridDim;
+inline thread_local dim3 threadIdx;
+inline int min(int a, int b) { return a < b ? a : b; }
+#define BHAH_PIN(x) ((void)0)
+#define __shared__ static
 inline hipError_t hipGetLastError() { return hipSuccess; }
 
 #define __global__
@@ -50,6 +54,55 @@ inline bhah_u4 __builtin_amdgcn_raw_buffer_load_b128(__amdgpu_buffer_rsrc_t r, u
   return v;
 }
 
+#ifdef BHAH_CPU_THREADED
+/* Kernels with workgroup barriers: every thread of a block runs as its own host
+ * thread and __syncthreads() is a barrier over the block's threads (blocks run
+ * one after another, so __shared__ arrays may be plain statics). */
+#include <condition_variable>
+#include <mutex>
+#include <thread>
+#include <vector>
+struct bhah_cpu_barrier_t {
+  std::mutex m;
+  std::condition_variable cv;
+  unsigned n = 0, count = 0, gen = 0;
+  void wait() {
+    std::unique_lock<std::mutex> l(m);
+    const unsigned g = gen;
+    if (++count == n) {
+      count = 0;
+      gen++;
+      cv.notify_all();
+    } else {
+      cv.wait(l, [&] { return g != gen; });
+    }
+  }
+};
+inline bhah_cpu_barrier_t bhah_cpu_bar;
+#define __syncthreads() bhah_cpu_bar.wait()
+
+template <typename K, typename... A>
+void cpu_launch(K kernel, dim3 g, dim3 b, A... args) {
+  gridDim = g;
+  blockDim = b;
+  bhah_cpu_bar.n = b.x * b.y * b.z;
+  for (unsigned bz = 0; bz < g.z; bz++)
+    for (unsigned by = 0; by < g.y; by++)
+      for (unsigned bx = 0; bx < g.x; bx++) {
+        blockIdx = dim3(bx, by, bz);
+        std::vector<std::thread> ts;
+        for (unsigned tz = 0; tz < b.z; tz++)
+          for (unsigned ty = 0; ty < b.y; ty++)
+            for (unsigned tx = 0; tx < b.x; tx++)
+              ts.emplace_back([=] {
+                threadIdx = dim3(tx, ty, tz);
+                kernel(args...);
+              });
+        for (auto &t : ts) t.join();
+      }
+}
+#else
+#define __syncthreads() __builtin_trap()
 template <typename K, typename... A>
 void cpu_launch(K kernel, dim3 g, dim3 b, A... args) {
   gridDim = g;
@@ -65,4 +118,5 @@ void cpu_launch(K kernel, dim3 g, dim3 b, A... args) {
               kernel(args...);
             }
 }
+#endif
 #define hipLaunchKernelGGL(k, g, b, shmem, stream, ...) cpu_launch(k, g, b, __VA_ARGS__)

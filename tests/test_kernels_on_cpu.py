@@ -44,7 +44,9 @@ def schedules(tag):
 def build(tag, kind, cand, workdir):
     src = os.path.join(GEN, f"bssn_{tag}_{kind}_{cand}.hip")
     exe = os.path.join(workdir, f"{kind}_{cand}")
-    cmd = ["g++", "-std=c++17", "-O1", "-g", "-fsanitize=address", "-fno-omit-frame-pointer", "-w",
+    # workgroup-tiled (LDS) schedules synchronise their threads: run each block's threads concurrently
+    threaded = ["-DBHAH_CPU_THREADED=1", "-pthread"] if re.search(r"_L\d", cand) else []
+    cmd = ["g++", "-std=c++17", "-O1", "-g", "-fsanitize=address", "-fno-omit-frame-pointer", "-w", *threaded,
            f"-I{SHIM}", f'-DKERNEL_FILE="{src}"', f"-DLAUNCH=launch_bssn_{kind}_{tag}_{cand}", DRIVER,
            "-o", exe, "-lm"]
     subprocess.run(cmd, check=True, capture_output=True)
