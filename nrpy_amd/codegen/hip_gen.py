@@ -379,7 +379,11 @@ def _ordered_body(spec: specs.Spec, fd_order: int, out_lvalues: List[str], ratio
     cname: Dict = {}
     if stream_linear:
         cores, terms = split_linear(spec)
-        if terms:
+        if terms and linear == "lds":
+            # upwind/KO contributions were accumulated by the caller (workgroup-tiled kernels)
+            for g in terms:
+                acc_names[g] = f"VPT_lin_{g}"
+        elif terms:
             coefs = sorted({c for tl in terms.values() for _, c in tl}, key=sp.default_sort_key)
             crep, cred = sp.cse(coefs, symbols=sp.numbered_symbols("lc"), order="none")
             for sy, e in crep:
@@ -921,10 +925,13 @@ def _group_bounds(lb: str, k: int, ngroups: int) -> str:
 # Candidate schedules per build: (kind, partition, launch bounds).
 FULL_CANDIDATES = {
     "rhs": ["p1_lb1", "p2_lb1_sj_lf_pf12_u32_ck8", "p2_lb1_sj_lf_pf12_u32_ck8_w2", "p2_lb1_sj_lf_pf16_u32_ck8_w2",
-            "p2_lb1_sj_lf_pf16_u32_ck6_w2", "p2_lb1_sj_lf_pf16_u32_w2"],
-    "ricci": ["r1_lb1", "r2_lb1_sj_pf32", "r2_lb1_sj_lf_pf16_zs16", "r2_lb1_sj_lf_pf32", "r2_lb1_sj_lf_pf12_zs16",
-              "r2_lb1_sj_lf_pf16_u32_w2", "r1_lb1_sj_L16x4x4x1", "r2_lb1_sj_L16x4x4x1", "r2_lb1_sj_L32x4x4x2",
-              "r3_lb2_sj_L16x4x4x1"],
+            "p2_lb1_sj_lf_pf16_u32_ck6_w2", "p1_lb1_sj_L16x4x4x1_d2_xcd", "p1_lb1_sj_L16x4x4x1_d3_xcd",
+            "p1_lb1_sj_L16x4x4x1_d4_xcd", "p1_lb1_sj_L16x4x4x1_d2_xcd_zf", "p1_lb1_sj_L16x4x4x1_d3_xcd_zf",
+            "p1_lb1_sj_L16x8x2x1_d3_xcd", "p1_lb1_sj_L8x4x8x1_d3_xcd_zf", "p1_lb1_sj_L32x4x2x1_d2_xcd",
+            "p1_lb1_sj_L16x2x8x1_d3_xcd_zf"],
+    "ricci": ["r1_lb1", "r2_lb1_sj_lf_pf12_zs16", "r1_lb1_sj_L16x4x4x1", "r1_lb1_sj_L16x4x4x1_d2_xcd",
+              "r1_lb1_sj_L16x8x2x1_d2_xcd", "r1_lb1_sj_L16x4x4x1_d2_xcd_zf", "r1_lb1_sj_L16x8x2x1_d3_xcd",
+              "r1_lb1_sj_L16x4x4x1_d3_xcd_zf", "r1_lb1_sj_L8x4x8x1_d2_xcd_zf", "r1_lb1_sj_L16x2x8x1_d2_xcd_zf"],
 }
 SMALL_CANDIDATES = {"rhs": ["p1_lb1", "p1_lb1_sl", "p3_lb1_sl", "p1_lb1_sl_ln_pf16", "p3_lb1_sl_ln_pf16",
                             "p3_lb1_sl_ln_pf16_u32", "p3_lb1_sl_ln_pf16_u32_ck8", "p3_lb1_sl_lf_pf16_u32_ck8",
@@ -961,6 +968,26 @@ class _LdsEmitter(StencilEmitter):
         return f"sh[{self.centre}{f' + {o}' if o > 0 else (f' - {-o}' if o < 0 else '')}]"
 
 
+def _lds_tile_order(xcd: bool, zfirst: bool) -> str:
+    """Tile of this workgroup.  Workgroups are dispatched in linear order, round-robin
+    over the 8 XCDs; with ``xcd`` XCD k takes the k-th contiguous eighth of the tiles
+    (neighbouring tiles, whose halo boxes overlap, then share an L2), and with
+    ``zfirst`` consecutive tiles run along xx2 first (their boxes overlap in 2H of
+    TZ + 2H planes)."""
+    if not (xcd or zfirst):
+        return "  const unsigned bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;"
+    lin = ("((n & 7u) == 0u) ? (h & 7u) * (n >> 3) + (h >> 3) : h" if xcd else "h")
+    dec = ("bz = l % gz;\n    bx = (l / gz) % gx;\n    by = l / (gz * gx);" if zfirst else
+           "bx = l % gx;\n    by = (l / gx) % gy;\n    bz = l / (gx * gy);")
+    return f"""  unsigned bx, by, bz;
+  {{
+    const unsigned gx = gridDim.x, gy = gridDim.y, gz = gridDim.z, n = gx * gy * gz;
+    const unsigned h = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const unsigned l = {lin};
+    {dec}
+  }}"""
+
+
 def is_lds(cand: str) -> bool:
     """Whether a schedule name selects the workgroup-tiled (LDS) kernels."""
     return any(o[:1] == "L" and o[1:2].isdigit() for o in cand.split("_"))
@@ -976,10 +1003,12 @@ def _lds_geometry(tok: str) -> Tuple[int, int, int, int]:
 
 
 def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str], geom: Tuple[int, int, int, int],
-               min_waves: int, scope: str, lv: List[str]) -> Tuple[str, str, int]:
+               min_waves: int, scope: str, lv: List[str], xcd: bool = False,
+               depth: int = 1, zfirst: bool = False) -> Tuple[str, str, int]:
     """One workgroup-tiled kernel: each gridfunction with derivatives is staged, one at a
     time, from global memory into a halo box in LDS (the next one's global loads in flight
-    while the current one's stencils read LDS), every derivative of the thread's points is
+    while the current one's stencils read LDS; ``depth`` gridfunctions ahead), every
+    derivative of the thread's points is
     formed from the box, then the point algebra runs with all inputs in registers.
 
     Stencil sums are the same expressions, in the same order, as the global-load kernels.
@@ -993,11 +1022,23 @@ def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str],
         used |= {str(x) for x in e.free_symbols}
     for _, e in sub.used_defs():
         used |= {str(x) for x in e.free_symbols}
-    dused = [d for d in derivs if d.symbol in used]
-    if any(d.op in ("dupD", "dKOD") for d in dused):
-        raise ValueError("LDS kernels take centred derivatives only")
-    H = fd_order // 2
-    staged = [g for g in sub.gfs if any(d.gf == g for d in dused)]
+    dused = [d for d in derivs if d.symbol in used and d.op in ("dD", "dDD")]
+    # upwind and KO derivatives enter linearly: accumulated per output while their
+    # gridfunction is in LDS, with coefficients formed before the staging
+    terms = split_linear(sub)[1] if any(d.op in ("dupD", "dKOD") for d in derivs) else {}
+    coefs = sorted({c for tl in terms.values() for _, c in tl}, key=sp.default_sort_key)
+    cidx = {c: j for j, c in enumerate(coefs)}
+    line_cf: Dict[Tuple[str, int], Dict[str, int]] = {}
+    line_out: Dict[Tuple[str, int], str] = {}
+    for g_out, tl in terms.items():
+        for d, c in terms_sorted(tl):
+            key = (d.gf, d.dirs[0])
+            if line_out.get(key, g_out) != g_out:
+                raise ValueError(f"{d.symbol} enters two outputs")
+            line_cf.setdefault(key, {})["up" if d.op == "dupD" else "ko"] = cidx[c]
+            line_out[key] = g_out
+    H = fd.ghost_zones(fd_order) if terms else fd_order // 2
+    staged = [g for g in sub.gfs if any(d.gf == g for d in dused) or any(k[0] == g for k in line_out)]
     BX, BY, BZ = TX + 2 * H, TY + 2 * H, TZ + 2 * H
     NB = BX * BY * BZ
     NL = (NB + NT - 1) // NT
@@ -1008,9 +1049,13 @@ def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str],
             ovr[n] = f"VPT_{n}" if n in staged else f"in[{sub.gfs.index(n)} * Ntot + idx]"
     for d in dused:
         ovr[d.symbol] = f"VPT_{d.symbol}"
-    body = _ordered_body(sub, fd_order, lv, rats, cse_scope=scope, linear="stream", prefetch=0, barriers=False,
+    body = _ordered_body(sub, fd_order, lv, rats, cse_scope=scope, linear="lds", prefetch=0, barriers=False,
                          defs_override=ovr)
-    flops = FLOPS[f"{sub.name}_fd{fd_order}{_mode_suffix('stream', 0)}"]
+    flops = FLOPS[f"{sub.name}_fd{fd_order}{_mode_suffix('lds', 0)}"]
+    pr = emit.DoublePrinter(rats)
+    w_up = dict(fd.weights_1d("dupD", fd_order))
+    w_dn = dict(fd.weights_1d("ddnD", fd_order))
+    w_ko = dict(fd.weights_1d("dKOD", fd_order))
     L: List[str] = []
     a = L.append
     a(f"""
@@ -1020,44 +1065,79 @@ extern "C" __global__ void __launch_bounds__({NT}{f', {min_waves}' if min_waves 
   // Tile of {TX}x{TY}x{TZ} interior points; halo box {BX}x{BY}x{BZ} = {NB} values per gridfunction.
   constexpr int BX = {BX}, BY = {BY}, NB = {NB}, NT = {NT}, NL = {NL};
   __shared__ double sh[NB];
+{_lds_tile_order(xcd, zfirst)}
   const int tid = (int)threadIdx.x;
   const int lx = tid % {TX}, ly = (tid / {TX}) % {TY}, lz = tid / {TX * TY};
   const long s1 = p.s1, s2 = p.s2, Ntot = p.Ntot;
-  const int o0 = p.NG + (int)blockIdx.x * {TX}, o1 = p.NG + (int)blockIdx.y * {TY}, o2 = p.NG + (int)blockIdx.z * {TZ};
+  const int o0 = p.NG + (int)bx * {TX}, o1 = p.NG + (int)by * {TY}, o2 = p.NG + (int)bz * {TZ};
   // Box element e of this thread's j-th slot; indices are clamped into the array, so
   // partial tiles at the upper interior edge read valid (unused) points.
   int ofs[NL];
 #pragma unroll
   for (int j = 0; j < NL; j++) {{
     const int e = tid + NT * j;
-    const int bx = e % BX, by = (e / BX) % BY, bz = e / (BX * BY);
-    const int g0 = min(o0 - {H} + bx, p.Nx0 - 1), g1 = min(o1 - {H} + by, p.Nx1 - 1), g2 = min(o2 - {H} + bz, p.Nx2 - 1);
+    const int ex = e % BX, ey = (e / BX) % BY, ez = e / (BX * BY);
+    const int g0 = min(o0 - {H} + ex, p.Nx0 - 1), g1 = min(o1 - {H} + ey, p.Nx1 - 1), g2 = min(o2 - {H} + ez, p.Nx2 - 1);
     ofs[j] = g0 + (int)s1 * g1 + (int)s2 * g2;
   }}""")
     for k in range(P):
         a(f"  const int c{k} = (lx + {H}) + BX * (ly + {H}) + BX * BY * (lz + {k * ZS + H});")
-    a("  double pf[NL];")
+    if terms:
+        # coefficients of the linear terms and the upwind control vector at each point
+        crep, cred = sp.cse(coefs, symbols=sp.numbered_symbols("lc"), order="none")
+        flops += sum(emit._flops(e) for _, e in crep) + sum(emit._flops(e) for e in cred)
+        upw = [sp.sympify(sub.upwind[d]) for d in range(3)]
+        names = set()
+        for e in [e for _, e in crep] + list(cred) + upw:
+            names |= {str(x) for x in e.free_symbols}
+        for k in range(P):
+            a(f"  double L{k}_coef[{len(coefs)}], L{k}_beta[3];")
+            a("  {")
+            a(f"    const int i0 = min(o0 + lx, p.Nx0 - p.NG - 1), i1 = min(o1 + ly, p.Nx1 - p.NG - 1);")
+            a(f"    const long idx = i0 + s1 * i1 + s2 * min(o2 + lz + {k * ZS}, p.Nx2 - p.NG - 1);")
+            for n in sorted(names):
+                if n in sub.gfs:
+                    a(f"    const double {n} = in[{sub.gfs.index(n)} * Ntot + idx];")
+                elif n in specs.RFM_SYMBOLS:
+                    a(f"    const double {n} = {RFM_EXPR[specs.RFM_SYMBOLS.index(n)]};")
+                elif n in PARAM_EXPR:
+                    a(f"    const double {n} = {PARAM_EXPR[n]};")
+                elif not n.startswith("lc"):
+                    raise ValueError(f"linear coefficient depends on {n}")
+            for sy, e in crep:
+                a(f"    const double {sy} = {pr.doprint(e)};")
+            for j, e in enumerate(cred):
+                a(f"    L{k}_coef[{j}] = {pr.doprint(e)};")
+            for d in range(3):
+                a(f"    L{k}_beta[{d}] = {pr.doprint(upw[d])};")
+            a("  }")
+            for g_out in sorted(terms):
+                a(f"  double V{k}_lin_{g_out} = 0.0;")
+    depth = max(1, min(depth, len(staged)))
+    for q in range(depth):
+        a(f"  double pf{q}[NL];")
 
-    def fetch(g):
+    def fetch(g, q):
         gi = sub.gfs.index(g)
         a(f"""  {{
     const double *src = in + {gi} * Ntot;
 #pragma unroll
-    for (int j = 0; j < NL; j++) pf[j] = src[ofs[j]];
+    for (int j = 0; j < NL; j++) pf{q}[j] = src[ofs[j]];
   }}""")
 
-    fetch(staged[0])
+    for q in range(depth):
+        fetch(staged[q], q)
     for si, g in enumerate(staged):
         a(f"""  // ---- {g}
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < NL; j++) {{
     const int e = tid + NT * j;
-    if (e < NB) sh[e] = pf[j];
+    if (e < NB) sh[e] = pf{si % depth}[j];
   }}
   __syncthreads();""")
-        if si + 1 < len(staged):
-            fetch(staged[si + 1])
+        if si + depth < len(staged):
+            fetch(staged[si + depth], si % depth)
         for k in range(P):
             st = _LdsEmitter(sub, fd_order, f"c{k}", BX, BY)
             # each value is pinned to a register where it is formed: otherwise the
@@ -1070,6 +1150,26 @@ extern "C" __global__ void __launch_bounds__({NT}{f', {min_waves}' if min_waves 
                 if d.gf == g:
                     a(f"  double V{k}_{d.symbol} = {st.expr(d)};")
                     a(f"  BHAH_PIN(V{k}_{d.symbol});")
+            for dirn in range(3):
+                key = (g, dirn)
+                if key not in line_out:
+                    continue
+                cf = line_cf[key]
+                acc = f"V{k}_lin_{line_out[key]}"
+                pts = {o: st.load(g, tuple(o if q == dirn else 0 for q in range(3))) for o in range(-H, H + 1)}
+                tc = []
+                if "up" in cf:
+                    su = " + ".join(f"{_dbl(w)} * {pts[o]}" for o, w in sorted(w_up.items()))
+                    sd = " + ".join(f"{_dbl(w)} * {pts[o]}" for o, w in sorted(w_dn.items()))
+                    tc.append(f"L{k}_coef[{cf['up']}] * (p.invdxx{dirn} * ((L{k}_beta[{dirn}] > 0.0) ? ({su}) : ({sd})))")
+                    flops += 2 * (2 * len(w_up) - 1) + 2
+                if "ko" in cf:
+                    sk = [f"{_dbl(w_ko[0])} * {pts[0]}"]
+                    sk += [f"{_dbl(w_ko[o])} * ({pts[o]} + {pts[-o]})" for o in sorted(w_ko) if o > 0]
+                    tc.append(f"L{k}_coef[{cf['ko']}] * (p.invdxx{dirn} * ({' + '.join(sk)}))")
+                    flops += 3 * (len(w_ko) // 2) + 3
+                a(f"  {acc} += {' + '.join(tc)};")
+                a(f"  BHAH_PIN({acc});")
     for k in range(P):
         a(f"""  {{
     const int i0 = o0 + lx, i1 = o1 + ly, i2 = o2 + lz + {k * ZS};
@@ -1127,7 +1227,9 @@ def candidate_tu(v: specs.Variant, fd_order: int, kind: str, cand: str, ensemble
         if lds:
             outs = list(bssn.EVOL_GFS if kind == "rhs" else bssn.AUXEVOL_GFS)
             src, launch, fl = lds_kernel(sub, fd_order, kname, rats, lds, 2 if lb == "lb2" else 1, scope,
-                                         [f"out[{outs.index(g)} * Ntot + idx]" for g, _ in sub.outputs])
+                                         [f"out[{outs.index(g)} * Ntot + idx]" for g, _ in sub.outputs], xcd=xcd,
+                                         depth=next((int(o[1:]) for o in opts if o[0] == "d" and o[1:].isdigit()), 1),
+                                         zfirst="zf" in opts)
             kernels.append(src)
             calls.append(launch)
             total += fl
