@@ -924,14 +924,12 @@ def _group_bounds(lb: str, k: int, ngroups: int) -> str:
 
 # Candidate schedules per build: (kind, partition, launch bounds).
 FULL_CANDIDATES = {
-    "rhs": ["p1_lb1", "p2_lb1_sj_lf_pf12_u32_ck8", "p2_lb1_sj_lf_pf12_u32_ck8_w2", "p2_lb1_sj_lf_pf16_u32_ck8_w2",
-            "p2_lb1_sj_lf_pf16_u32_ck6_w2", "p1_lb1_sj_L16x4x4x1_d2_xcd", "p1_lb1_sj_L16x4x4x1_d3_xcd",
-            "p1_lb1_sj_L16x4x4x1_d4_xcd", "p1_lb1_sj_L16x4x4x1_d2_xcd_zf", "p1_lb1_sj_L16x4x4x1_d3_xcd_zf",
-            "p1_lb1_sj_L16x8x2x1_d3_xcd", "p1_lb1_sj_L8x4x8x1_d3_xcd_zf", "p1_lb1_sj_L32x4x2x1_d2_xcd",
-            "p1_lb1_sj_L16x2x8x1_d3_xcd_zf"],
-    "ricci": ["r1_lb1", "r2_lb1_sj_lf_pf12_zs16", "r1_lb1_sj_L16x4x4x1", "r1_lb1_sj_L16x4x4x1_d2_xcd",
-              "r1_lb1_sj_L16x8x2x1_d2_xcd", "r1_lb1_sj_L16x4x4x1_d2_xcd_zf", "r1_lb1_sj_L16x8x2x1_d3_xcd",
-              "r1_lb1_sj_L16x4x4x1_d3_xcd_zf", "r1_lb1_sj_L8x4x8x1_d2_xcd_zf", "r1_lb1_sj_L16x2x8x1_d2_xcd_zf"],
+    "rhs": ["p1_lb1", "p2_lb1_sj_lf_pf16_u32_ck6_w2", "p1_lb1_sj_L16x4x4x1_d2_xcd", "p1_lb1_sj_L16x4x4x1_g2_d1_xcd",
+            "p1_lb1_sj_L16x4x4x1_g2_d2_xcd", "p1_lb1_sj_L16x4x4x1_g3_d1_xcd", "p1_lb1_sj_L16x4x4x1_g4_d1_xcd",
+            "p1_lb1_sj_L16x8x2x1_g2_d1_xcd"],
+    "ricci": ["r1_lb1", "r2_lb1_sj_lf_pf12_zs16", "r1_lb1_sj_L16x8x2x1_d3_xcd", "r1_lb1_sj_L16x4x4x1_d2_xcd",
+              "r1_lb1_sj_L16x8x2x1_g3_d1_xcd", "r1_lb1_sj_L16x8x2x1_g2_d2_xcd", "r1_lb1_sj_L16x4x4x1_g3_d1_xcd",
+              "r1_lb1_sj_L16x4x4x1_g3_d2_xcd"],
 }
 SMALL_CANDIDATES = {"rhs": ["p1_lb1", "p1_lb1_sl", "p3_lb1_sl", "p1_lb1_sl_ln_pf16", "p3_lb1_sl_ln_pf16",
                             "p3_lb1_sl_ln_pf16_u32", "p3_lb1_sl_ln_pf16_u32_ck8", "p3_lb1_sl_lf_pf16_u32_ck8",
@@ -959,12 +957,12 @@ LDS_PIN_MACRO = """#ifndef BHAH_PIN
 class _LdsEmitter(StencilEmitter):
     """Stencil expressions reading a workgroup's LDS box ``sh`` around box index ``c``."""
 
-    def __init__(self, spec: specs.Spec, fd_order: int, centre: str, BX: int, BY: int):
+    def __init__(self, spec: specs.Spec, fd_order: int, centre: str, BX: int, BY: int, base: int = 0):
         super().__init__(spec, fd_order)
-        self.centre, self.BX, self.BY = centre, BX, BY
+        self.centre, self.BX, self.BY, self.base = centre, BX, BY, base
 
     def load(self, gf: str, off: Tuple[int, int, int]) -> str:
-        o = off[0] + self.BX * off[1] + self.BX * self.BY * off[2]
+        o = self.base + off[0] + self.BX * off[1] + self.BX * self.BY * off[2]
         return f"sh[{self.centre}{f' + {o}' if o > 0 else (f' - {-o}' if o < 0 else '')}]"
 
 
@@ -1004,10 +1002,11 @@ def _lds_geometry(tok: str) -> Tuple[int, int, int, int]:
 
 def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str], geom: Tuple[int, int, int, int],
                min_waves: int, scope: str, lv: List[str], xcd: bool = False,
-               depth: int = 1, zfirst: bool = False) -> Tuple[str, str, int]:
+               depth: int = 1, zfirst: bool = False, group: int = 1) -> Tuple[str, str, int]:
     """One workgroup-tiled kernel: each gridfunction with derivatives is staged, one at a
     time, from global memory into a halo box in LDS (the next one's global loads in flight
-    while the current one's stencils read LDS; ``depth`` gridfunctions ahead), every
+    while the current one's stencils read LDS; ``group`` gridfunctions per barrier-separated
+    phase, ``depth`` phases ahead), every
     derivative of the thread's points is
     formed from the box, then the point algebra runs with all inputs in registers.
 
@@ -1064,7 +1063,7 @@ extern "C" __global__ void __launch_bounds__({NT}{f', {min_waves}' if min_waves 
         const double *__restrict__ rfm0, const double *__restrict__ rfm1, const bhah_kparams p) {{
   // Tile of {TX}x{TY}x{TZ} interior points; halo box {BX}x{BY}x{BZ} = {NB} values per gridfunction.
   constexpr int BX = {BX}, BY = {BY}, NB = {NB}, NT = {NT}, NL = {NL};
-  __shared__ double sh[NB];
+  __shared__ double sh[{min(group, len(staged))} * NB];
 {_lds_tile_order(xcd, zfirst)}
   const int tid = (int)threadIdx.x;
   const int lx = tid % {TX}, ly = (tid / {TX}) % {TY}, lz = tid / {TX * TY};
@@ -1113,63 +1112,72 @@ extern "C" __global__ void __launch_bounds__({NT}{f', {min_waves}' if min_waves 
             a("  }")
             for g_out in sorted(terms):
                 a(f"  double V{k}_lin_{g_out} = 0.0;")
-    depth = max(1, min(depth, len(staged)))
-    for q in range(depth):
+    phases = [staged[i:i + group] for i in range(0, len(staged), group)]
+    depth = max(1, min(depth, len(phases)))
+    nbuf = depth * group
+    for q in range(nbuf):
         a(f"  double pf{q}[NL];")
 
-    def fetch(g, q):
-        gi = sub.gfs.index(g)
-        a(f"""  {{
-    const double *src = in + {gi} * Ntot;
+    def fetch(ph):
+        for q, g in enumerate(phases[ph]):
+            a(f"""  {{
+    const double *src = in + {sub.gfs.index(g)} * Ntot;
 #pragma unroll
-    for (int j = 0; j < NL; j++) pf{q}[j] = src[ofs[j]];
+    for (int j = 0; j < NL; j++) pf{(ph % depth) * group + q}[j] = src[ofs[j]];
   }}""")
 
-    for q in range(depth):
-        fetch(staged[q], q)
-    for si, g in enumerate(staged):
-        a(f"""  // ---- {g}
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < NL; j++) {{
-    const int e = tid + NT * j;
-    if (e < NB) sh[e] = pf{si % depth}[j];
-  }}
-  __syncthreads();""")
-        if si + depth < len(staged):
-            fetch(staged[si + depth], si % depth)
-        for k in range(P):
-            st = _LdsEmitter(sub, fd_order, f"c{k}", BX, BY)
-            # each value is pinned to a register where it is formed: otherwise the
-            # compiler sinks the stencil sums into the guarded algebra below and keeps
-            # every raw LDS value alive across the staging barriers instead
-            if g in ovr and g in used:
-                a(f"  double V{k}_{g} = sh[c{k}];")
-                a(f"  BHAH_PIN(V{k}_{g});")
-            for d in dused:
-                if d.gf == g:
-                    a(f"  double V{k}_{d.symbol} = {st.expr(d)};")
-                    a(f"  BHAH_PIN(V{k}_{d.symbol});")
-            for dirn in range(3):
-                key = (g, dirn)
-                if key not in line_out:
-                    continue
-                cf = line_cf[key]
-                acc = f"V{k}_lin_{line_out[key]}"
-                pts = {o: st.load(g, tuple(o if q == dirn else 0 for q in range(3))) for o in range(-H, H + 1)}
-                tc = []
-                if "up" in cf:
-                    su = " + ".join(f"{_dbl(w)} * {pts[o]}" for o, w in sorted(w_up.items()))
-                    sd = " + ".join(f"{_dbl(w)} * {pts[o]}" for o, w in sorted(w_dn.items()))
-                    tc.append(f"L{k}_coef[{cf['up']}] * (p.invdxx{dirn} * ((L{k}_beta[{dirn}] > 0.0) ? ({su}) : ({sd})))")
-                    flops += 2 * (2 * len(w_up) - 1) + 2
-                if "ko" in cf:
-                    sk = [f"{_dbl(w_ko[0])} * {pts[0]}"]
-                    sk += [f"{_dbl(w_ko[o])} * ({pts[o]} + {pts[-o]})" for o in sorted(w_ko) if o > 0]
-                    tc.append(f"L{k}_coef[{cf['ko']}] * (p.invdxx{dirn} * ({' + '.join(sk)}))")
-                    flops += 3 * (len(w_ko) // 2) + 3
-                a(f"  {acc} += {' + '.join(tc)};")
-                a(f"  BHAH_PIN({acc});")
+    for ph in range(depth):
+        fetch(ph)
+    for ph, gfs in enumerate(phases):
+        a(f"  // ---- {', '.join(gfs)}")
+        a("  __syncthreads();")
+        a("#pragma unroll")
+        a("  for (int j = 0; j < NL; j++) {")
+        a("    const int e = tid + NT * j;")
+        a("    if (e < NB) {")
+        for q in range(len(gfs)):
+            a(f"      sh[{q} * NB + e] = pf{(ph % depth) * group + q}[j];")
+        a("    }")
+        a("  }")
+        a("  __syncthreads();")
+        if ph + depth < len(phases):
+            fetch(ph + depth)
+        for q, g in enumerate(gfs):
+            for k in range(P):
+                st = _LdsEmitter(sub, fd_order, f"c{k}", BX, BY, q * NB)
+                # each value is pinned to a register where it is formed: otherwise the
+                # compiler sinks the stencil sums into the guarded algebra below and keeps
+                # every raw LDS value alive across the staging barriers instead
+                if g in ovr and g in used:
+                    a(f"  double V{k}_{g} = {st.load(g, (0, 0, 0))};")
+                    a(f"  BHAH_PIN(V{k}_{g});")
+                for d in dused:
+                    if d.gf == g:
+                        a(f"  double V{k}_{d.symbol} = {st.expr(d)};")
+                        a(f"  BHAH_PIN(V{k}_{d.symbol});")
+                for dirn in range(3):
+                    key = (g, dirn)
+                    if key not in line_out:
+                        continue
+                    cf = line_cf[key]
+                    acc = f"V{k}_lin_{line_out[key]}"
+                    pts = {o: st.load(g, tuple(o if r == dirn else 0 for r in range(3))) for o in range(-H, H + 1)}
+                    tc = []
+                    if "up" in cf:
+                        su = " + ".join(f"{_dbl(w)} * {pts[o]}" for o, w in sorted(w_up.items()))
+                        sd = " + ".join(f"{_dbl(w)} * {pts[o]}" for o, w in sorted(w_dn.items()))
+                        tc.append(f"L{k}_coef[{cf['up']}] * (p.invdxx{dirn} * ((L{k}_beta[{dirn}] > 0.0) ? ({su}) : "
+                                  f"({sd})))")
+                        if k == 0:
+                            flops += 2 * (2 * len(w_up) - 1) + 2
+                    if "ko" in cf:
+                        sk = [f"{_dbl(w_ko[0])} * {pts[0]}"]
+                        sk += [f"{_dbl(w_ko[o])} * ({pts[o]} + {pts[-o]})" for o in sorted(w_ko) if o > 0]
+                        tc.append(f"L{k}_coef[{cf['ko']}] * (p.invdxx{dirn} * ({' + '.join(sk)}))")
+                        if k == 0:
+                            flops += 3 * (len(w_ko) // 2) + 3
+                    a(f"  {acc} += {' + '.join(tc)};")
+                    a(f"  BHAH_PIN({acc});")
     for k in range(P):
         a(f"""  {{
     const int i0 = o0 + lx, i1 = o1 + ly, i2 = o2 + lz + {k * ZS};
@@ -1229,7 +1237,8 @@ def candidate_tu(v: specs.Variant, fd_order: int, kind: str, cand: str, ensemble
             src, launch, fl = lds_kernel(sub, fd_order, kname, rats, lds, 2 if lb == "lb2" else 1, scope,
                                          [f"out[{outs.index(g)} * Ntot + idx]" for g, _ in sub.outputs], xcd=xcd,
                                          depth=next((int(o[1:]) for o in opts if o[0] == "d" and o[1:].isdigit()), 1),
-                                         zfirst="zf" in opts)
+                                         zfirst="zf" in opts,
+                                         group=next((int(o[1:]) for o in opts if o[0] == "g" and o[1:].isdigit()), 1))
             kernels.append(src)
             calls.append(launch)
             total += fl
