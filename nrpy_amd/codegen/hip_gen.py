@@ -1302,22 +1302,40 @@ def bssn_kernels(v: specs.Variant, fd_order: int, ensemble: bool = False) -> str
     ngf = len(bssn.EVOL_GFS)
     # ---- RK4 substep update + det(gammabar) enforcement, all points, one streaming pass
     det = detgamma_inline(v, rats, "nxt")
+    # One thread per point in array order (every lane of every wave on consecutive
+    # addresses, no partial rows); every array is streamed once, so loads and stores
+    # carry the non-temporal hint.
+    rk_nt = RK_EPILOGUE
+    for a, b in (("y_run[g * Ntot + idx] = (1.0 / 6.0) * dt * k;",
+                  "__builtin_nontemporal_store((1.0 / 6.0) * dt * k, &y_run[g * Ntot + idx]);"),
+                 ("(1.0 / 6.0) * dt * k + y_n[g * Ntot + idx] + y_run[g * Ntot + idx];",
+                  "(1.0 / 6.0) * dt * k + __builtin_nontemporal_load(&y_n[g * Ntot + idx]) + "
+                  "__builtin_nontemporal_load(&y_run[g * Ntot + idx]);"),
+                 ("y_run[g * Ntot + idx] = (1.0 / 3.0) * dt * k + y_run[g * Ntot + idx];",
+                  "__builtin_nontemporal_store((1.0 / 3.0) * dt * k + __builtin_nontemporal_load(&y_run[g * Ntot + idx]), "
+                  "&y_run[g * Ntot + idx]);"),
+                 ("a * dt * k + y_n[g * Ntot + idx];", "a * dt * k + __builtin_nontemporal_load(&y_n[g * Ntot + idx]);")):
+        assert a in rk_nt, a
+        rk_nt = rk_nt.replace(a, b)
     rkdet = f"""
 extern "C" __global__ void __launch_bounds__(256)
 bssn_rk_det_{tag}(const double *__restrict__ kin, const double *y_n, double *__restrict__ y_run, double *out,
                   const double *__restrict__ rfm0, const double *__restrict__ rfm1, const bhah_kparams p) {{
-{_prologue(False)}
+  const long Ntot = p.Ntot;
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= Ntot) return;
+  const int i0 = (int)(idx % p.Nx0), i1 = (int)((idx / p.Nx0) % p.Nx1);
   constexpr int NGF = {ngf};
   double kk[NGF], ctr[NGF];
 #pragma unroll
   for (int g = 0; g < NGF; g++) {{
-    kk[g] = kin[g * Ntot + idx];
-    ctr[g] = (p.substep == 1) ? y_n[g * Ntot + idx] : 0.0;
+    kk[g] = __builtin_nontemporal_load(&kin[g * Ntot + idx]);
+    ctr[g] = (p.substep == 1) ? __builtin_nontemporal_load(&y_n[g * Ntot + idx]) : 0.0;
   }}
-{RK_EPILOGUE}
+{rk_nt}
 {det}
 #pragma unroll
-  for (int g = 0; g < NGF; g++) out[g * Ntot + idx] = nxt[g];
+  for (int g = 0; g < NGF; g++) __builtin_nontemporal_store(nxt[g], &out[g * Ntot + idx]);
 }}
 """
     ens_rk = ""
@@ -1386,7 +1404,10 @@ bssn_constraints_{tag}(const double *__restrict__ in, const double *__restrict__
 // Host-side launchers (one translation unit per variant; no relocatable device code needed).
 extern "C" hipError_t launch_bssn_rk_det_{tag}(dim3 g, dim3 b, hipStream_t s, const double *kin, const double *y_n,
     double *y_run, double *out, const double *rfm0, const double *rfm1, bhah_kparams p) {{
-  hipLaunchKernelGGL(bssn_rk_det_{tag}, g, b, 0, s, kin, y_n, y_run, out, rfm0, rfm1, p);
+  (void)g;
+  (void)b;
+  hipLaunchKernelGGL(bssn_rk_det_{tag}, dim3((unsigned)((p.Ntot + 255) / 256)), dim3(256), 0, s, kin, y_n, y_run, out,
+                     rfm0, rfm1, p);
   return hipGetLastError();
 }}
 extern "C" hipError_t launch_bssn_constraints_{tag}(dim3 g, dim3 b, hipStream_t s, const double *in,
