@@ -236,21 +236,42 @@ bc_radiation_kernel(const int16_t *__restrict__ pts, int npts, int ngfs, const d
 
 // All ghost layers of the radiation condition in one launch, for grids whose pure outer
 // points are radial lines (i1, i2 fixed, i0 = Nx0-NG .. Nx0-1; spherical-like coordinates):
-// one thread per (line, gridfunction) walks the layers innermost first, so each layer reads
-// the previous layer's value it depends on from its own earlier store.
+// one thread per (line, gridfunction) walks the layers innermost first; the right-hand side
+// at a layer's interior neighbour is the value the thread formed one layer earlier (the
+// first layer's is loaded), kept in a register.  Same arithmetic as rad_point.
 __global__ void __launch_bounds__(256)
 bc_radiation_lines_kernel(const int *__restrict__ lines, int nlines, int ngfs, int NG, const double *__restrict__ gfs,
                           double *__restrict__ rhs, const double *__restrict__ xx0,
                           const double *__restrict__ wavespeed, const double *__restrict__ finf, RadStencil st,
                           int R, int coord, double AMPL, double SINHW, int Nx0, long s1, long s2, long Ntot,
                           double invdxx0) {
+#pragma clang fp contract(off)
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)nlines * ngfs) return;
   const int l = (int)(t % nlines), gf = (int)(t / nlines);
-  int16_t q[6] = {0, (int16_t)lines[2 * l], (int16_t)lines[2 * l + 1], -1, 0, 0};
+  const long base = s1 * lines[2 * l] + s2 * lines[2 * l + 1];
+  const double *f = &gfs[gf * Ntot];
+  double *rr = &rhs[gf * Ntot];
+  const double c = wavespeed[gf], f_inf = finf[gf];
+  double dtf_int = rr[base + Nx0 - NG - 1];
+  double r_int, dxr_int;
+  r_and_dx0dr(coord, AMPL, SINHW, xx0[Nx0 - NG - 1], r_int, dxr_int);
   for (int gz = 0; gz < NG; gz++) {
-    q[0] = (int16_t)(Nx0 - NG + gz);
-    rad_point(q, gf, gfs, rhs, xx0, wavespeed[gf], finf[gf], st, R, coord, AMPL, SINHW, Nx0, s1, s2, Ntot, invdxx0);
+    const int i0 = Nx0 - NG + gz, j0 = i0 - 1;
+    const long idx = base + i0, idx_int = base + j0;
+    double r, dxr;
+    r_and_dx0dr(coord, AMPL, SINHW, xx0[i0], r, dxr);
+    const double dfr = partial_r_f(st, R, f, Nx0, idx, i0, -1, dxr, invdxx0);
+    const double dfr_int = partial_r_f(st, R, f, Nx0, idx_int, j0, -1, dxr_int, invdxx0);
+    const double dtf_int_out = -c * (dfr_int + (f[idx_int] - f_inf) / r_int);
+    const double k = r_int * r_int * r_int * (dtf_int - dtf_int_out);
+    const double rinv = 1.0 / r;
+    const double dtf_out = -c * (dfr + (f[idx] - f_inf) * rinv);
+    const double v = dtf_out + k * rinv * rinv * rinv;
+    rr[idx] = v;
+    dtf_int = v;
+    r_int = r;
+    dxr_int = dxr;
   }
 }
 
@@ -311,12 +332,15 @@ bc_extrap_kernel(const int16_t *__restrict__ pts, int npts, int ngfs, double *__
 __global__ void __launch_bounds__(256)
 bc_inner_kernel(const int *__restrict__ dst, const int *__restrict__ src, const unsigned short *__restrict__ mask,
                 int npts, int ngfs, const int *__restrict__ gf_parity, double *__restrict__ gfs, long Ntot) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)npts * ngfs) return;
-  const int p = (int)(t % npts), gf = (int)(t / npts);
-  double *f = &gfs[gf * Ntot];
-  const double v = f[src[p]];
-  f[dst[p]] = ((mask[p] >> gf_parity[gf]) & 1u) ? -v : v;
+  // one thread per boundary point, all gridfunctions: the point's record is read once
+  const int p = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (p >= npts) return;
+  const long d = dst[p], sr = src[p];
+  const unsigned m = mask[p];
+  for (int gf = 0; gf < ngfs; gf++) {
+    const double v = gfs[gf * Ntot + sr];
+    gfs[gf * Ntot + d] = ((m >> gf_parity[gf]) & 1u) ? -v : v;
+  }
 }
 
 // ---- Adjoints of the boundary conditions (reverse mode; linear maps).
@@ -970,7 +994,7 @@ int bhah_hip_apply_bcs_inner_only(bhah_patch *P, double *gfs) {
   const int n = (int)P->inner_dst.size();
   if (n == 0) return BHAH_OK;
   dim3 g, b;
-  launch_1d((long)n * P->ngfs, g, b);
+  launch_1d((long)n, g, b);
   hipLaunchKernelGGL(bc_inner_kernel, g, b, 0, P->stream, P->d_inner_dst, P->d_inner_src, P->d_inner_mask, n, P->ngfs,
                      P->d_gf_parity, gfs, P->Ntot);
   HIP_TRY(hipGetLastError());
