@@ -97,7 +97,7 @@ def tune(cfg, reps: int = 5, write: bool = True, log=print) -> Dict:
     import torch
 
     from . import initial_data as ID
-    from .runtime import KERNEL_RHS, KERNEL_RICCI, Patch
+    from .runtime import KERNEL_RHS, KERNEL_RICCI, BHaHError, Patch
 
     P = Patch(cfg)
     xx = P.xx()
@@ -113,7 +113,11 @@ def tune(cfg, reps: int = 5, write: bool = True, log=print) -> Dict:
         fn = (lambda: P.Ricci_eval(yd, aux)) if kind == KERNEL_RICCI else (lambda: P.rhs_eval(aux, yd, rhs))
         res = {}
         for name in P.schedules(kind):
-            P.select_schedule(kind, name)
+            try:
+                P.select_schedule(kind, name)
+            except BHaHError as e:  # schedule not usable on this grid
+                log(f"{label} {name}: skipped ({e})")
+                continue
             res[name] = _time(fn, stream, reps)
             timings[f"{label}:{name}"] = res[name]
             log(f"{label} {name}: {res[name]:.3f} ms")

@@ -926,10 +926,12 @@ def _group_bounds(lb: str, k: int, ngroups: int) -> str:
 FULL_CANDIDATES = {
     "rhs": ["p1_lb1", "p2_lb1_sj_lf_pf16_u32_ck6_w2", "p1_lb1_sj_L16x4x4x1_d2_xcd", "p1_lb1_sj_L16x4x4x1_g2_d1_xcd",
             "p1_lb1_sj_L16x4x4x1_g2_d2_xcd", "p1_lb1_sj_L16x4x4x1_g3_d1_xcd", "p1_lb1_sj_L16x4x4x1_g4_d1_xcd",
-            "p1_lb1_sj_L16x8x2x1_g2_d1_xcd"],
+            "p1_lb1_sj_L16x8x2x1_g2_d1_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_xcd", "p1_lb1_sj_L16x4x4x1_d4_dma_xcd",
+            "p1_lb1_sj_L16x4x4x1_d5_dma_xcd", "p1_lb1_sj_L16x8x2x1_d3_dma_xcd"],
     "ricci": ["r1_lb1", "r2_lb1_sj_lf_pf12_zs16", "r1_lb1_sj_L16x8x2x1_d3_xcd", "r1_lb1_sj_L16x4x4x1_d2_xcd",
               "r1_lb1_sj_L16x8x2x1_g3_d1_xcd", "r1_lb1_sj_L16x8x2x1_g2_d2_xcd", "r1_lb1_sj_L16x4x4x1_g3_d1_xcd",
-              "r1_lb1_sj_L16x4x4x1_g3_d2_xcd"],
+              "r1_lb1_sj_L16x4x4x1_g3_d2_xcd", "r1_lb1_sj_L16x4x4x1_d3_dma_xcd", "r1_lb1_sj_L16x4x4x1_g3_d2_dma_xcd",
+              "r1_lb1_sj_L16x4x4x1_d5_dma_xcd", "r1_lb1_sj_L16x8x2x1_g3_d2_dma_xcd"],
 }
 SMALL_CANDIDATES = {"rhs": ["p1_lb1", "p1_lb1_sl", "p3_lb1_sl", "p1_lb1_sl_ln_pf16", "p3_lb1_sl_ln_pf16",
                             "p3_lb1_sl_ln_pf16_u32", "p3_lb1_sl_ln_pf16_u32_ck8", "p3_lb1_sl_lf_pf16_u32_ck8",
@@ -986,6 +988,42 @@ def _lds_tile_order(xcd: bool, zfirst: bool) -> str:
   }}"""
 
 
+LDS_DMA_MACROS = """#ifndef BHAH_DMA16
+// 16 bytes per lane straight from global memory into LDS at dst + 16 * lane (dst uniform per wave)
+#define BHAH_DMA16(src, dst) __builtin_amdgcn_global_load_lds((const void *)(src), (__attribute__((address_space(3))) void *)(dst), 16, 0, 0)
+// wait until at most n vector-memory operations of this wave are outstanding (gfx9 encoding)
+#define BHAH_VMWAIT(n) __builtin_amdgcn_s_waitcnt(((n) & 15) | (((n) >> 4) << 14) | 0x0F70)
+// workgroup barrier without a memory fence (LDS reads of this wave retired first)
+#define BHAH_BARRIER() asm volatile("s_waitcnt lgkmcnt(0)\\n\\ts_barrier" ::: "memory")
+#endif
+"""
+
+
+def _lds_offsets(dma: bool, H: int, SH: int, NJ: int, NL: int) -> str:
+    """Per-thread global offsets of the box elements it loads.  Indices are clamped into
+    the array, so partial tiles at the upper interior edge read valid (unused) points."""
+    if not dma:
+        return f"""  int ofs[NL];
+#pragma unroll
+  for (int j = 0; j < NL; j++) {{
+    const int e = tid + NT * j;
+    const int ex = e % BX, ey = (e / BX) % BY, ez = e / (BX * BY);
+    const int g0 = min(o0 - {H} + ex, p.Nx0 - 1), g1 = min(o1 - {H} + ey, p.Nx1 - 1), g2 = min(o2 - {H} + ez, p.Nx2 - 1);
+    ofs[j] = g0 + (int)s1 * g1 + (int)s2 * g2;
+  }}"""
+    # pair j of this thread: box elements 2q, 2q + 1 (q = tid + NT j) are xx0 neighbours
+    # (BX is even); a pair start past Nx0 - 2 only feeds points outside the interior
+    return f"""  const int wv = tid >> 6;
+  int pofs[{NJ}];
+#pragma unroll
+  for (int j = 0; j < {NJ}; j++) {{
+    const int e = 2 * (tid + NT * j);
+    const int ex = e % BX, ey = (e / BX) % BY, ez = e / (BX * BY);
+    const int g0 = min(o0 - {H + SH} + ex, p.Nx0 - 2), g1 = min(o1 - {H} + ey, p.Nx1 - 1), g2 = min(o2 - {H} + ez, p.Nx2 - 1);
+    pofs[j] = g0 + (int)s1 * g1 + (int)s2 * g2;
+  }}"""
+
+
 def is_lds(cand: str) -> bool:
     """Whether a schedule name selects the workgroup-tiled (LDS) kernels."""
     return any(o[:1] == "L" and o[1:2].isdigit() for o in cand.split("_"))
@@ -1002,13 +1040,20 @@ def _lds_geometry(tok: str) -> Tuple[int, int, int, int]:
 
 def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str], geom: Tuple[int, int, int, int],
                min_waves: int, scope: str, lv: List[str], xcd: bool = False,
-               depth: int = 1, zfirst: bool = False, group: int = 1) -> Tuple[str, str, int]:
+               depth: int = 1, zfirst: bool = False, group: int = 1, dma: bool = False) -> Tuple[str, str, int]:
     """One workgroup-tiled kernel: each gridfunction with derivatives is staged, one at a
     time, from global memory into a halo box in LDS (the next one's global loads in flight
     while the current one's stencils read LDS; ``group`` gridfunctions per barrier-separated
     phase, ``depth`` phases ahead), every
     derivative of the thread's points is
     formed from the box, then the point algebra runs with all inputs in registers.
+
+    With ``dma`` the boxes are filled by direct global-to-LDS loads (16 bytes, two xx0
+    neighbours, per lane) into a ring of ``depth`` phase buffers, so prefetching takes no
+    registers and no LDS store instructions; workgroup barriers are then plain s_barrier
+    with explicit vmcnt waits counted per phase.  The box's xx0 origin is rounded down
+    to an even index (every pair 16-byte aligned), which needs an even Nx0 (checked by the
+    host).
 
     Stencil sums are the same expressions, in the same order, as the global-load kernels.
     Returns (kernel source, launch statement, flops per point)."""
@@ -1038,9 +1083,15 @@ def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str],
             line_out[key] = g_out
     H = fd.ghost_zones(fd_order) if terms else fd_order // 2
     staged = [g for g in sub.gfs if any(d.gf == g for d in dused) or any(k[0] == g for k in line_out)]
-    BX, BY, BZ = TX + 2 * H, TY + 2 * H, TZ + 2 * H
+    # xx0 shift of the box origin that makes it even (DMA pairs): NG - H is fixed per build
+    SH = ((fd.ghost_zones(fd_order) - H) & 1) if dma else 0
+    if dma and TX % 2:
+        raise ValueError("DMA staging needs an even tile width")
+    BX, BY, BZ = TX + 2 * H + 2 * SH, TY + 2 * H, TZ + 2 * H
     NB = BX * BY * BZ
     NL = (NB + NT - 1) // NT
+    NJ = (NB // 2 + NT - 1) // NT  # 16-byte DMA loads per thread per box
+    SLOT = 2 * NT * NJ if dma else NB  # doubles per box (DMA boxes padded so every lane loads)
     # algebra for a placeholder point; each point substitutes its own register names
     ovr: Dict[str, str] = {}
     for n in centre:
@@ -1063,24 +1114,15 @@ extern "C" __global__ void __launch_bounds__({NT}{f', {min_waves}' if min_waves 
         const double *__restrict__ rfm0, const double *__restrict__ rfm1, const bhah_kparams p) {{
   // Tile of {TX}x{TY}x{TZ} interior points; halo box {BX}x{BY}x{BZ} = {NB} values per gridfunction.
   constexpr int BX = {BX}, BY = {BY}, NB = {NB}, NT = {NT}, NL = {NL};
-  __shared__ double sh[{min(group, len(staged))} * NB];
+  __shared__ double sh[{(max(1, min(depth, -(-len(staged) // group))) * group if dma else min(group, len(staged))) * SLOT}];
 {_lds_tile_order(xcd, zfirst)}
   const int tid = (int)threadIdx.x;
   const int lx = tid % {TX}, ly = (tid / {TX}) % {TY}, lz = tid / {TX * TY};
   const long s1 = p.s1, s2 = p.s2, Ntot = p.Ntot;
   const int o0 = p.NG + (int)bx * {TX}, o1 = p.NG + (int)by * {TY}, o2 = p.NG + (int)bz * {TZ};
-  // Box element e of this thread's j-th slot; indices are clamped into the array, so
-  // partial tiles at the upper interior edge read valid (unused) points.
-  int ofs[NL];
-#pragma unroll
-  for (int j = 0; j < NL; j++) {{
-    const int e = tid + NT * j;
-    const int ex = e % BX, ey = (e / BX) % BY, ez = e / (BX * BY);
-    const int g0 = min(o0 - {H} + ex, p.Nx0 - 1), g1 = min(o1 - {H} + ey, p.Nx1 - 1), g2 = min(o2 - {H} + ez, p.Nx2 - 1);
-    ofs[j] = g0 + (int)s1 * g1 + (int)s2 * g2;
-  }}""")
+{_lds_offsets(dma, H, SH, NJ, NL)}""")
     for k in range(P):
-        a(f"  const int c{k} = (lx + {H}) + BX * (ly + {H}) + BX * BY * (lz + {k * ZS + H});")
+        a(f"  const int c{k} = (lx + {H + SH}) + BX * (ly + {H}) + BX * BY * (lz + {k * ZS + H});")
     if terms:
         # coefficients of the linear terms and the upwind control vector at each point
         crep, cred = sp.cse(coefs, symbols=sp.numbered_symbols("lc"), order="none")
@@ -1114,37 +1156,17 @@ extern "C" __global__ void __launch_bounds__({NT}{f', {min_waves}' if min_waves 
                 a(f"  double V{k}_lin_{g_out} = 0.0;")
     phases = [staged[i:i + group] for i in range(0, len(staged), group)]
     depth = max(1, min(depth, len(phases)))
-    nbuf = depth * group
-    for q in range(nbuf):
-        a(f"  double pf{q}[NL];")
+    fl = [flops]
 
-    def fetch(ph):
-        for q, g in enumerate(phases[ph]):
-            a(f"""  {{
-    const double *src = in + {sub.gfs.index(g)} * Ntot;
-#pragma unroll
-    for (int j = 0; j < NL; j++) pf{(ph % depth) * group + q}[j] = src[ofs[j]];
-  }}""")
+    def box(ph, q):
+        """LDS offset (doubles) of box q of phase ph."""
+        return ((ph % depth) * group + q) * SLOT if dma else q * NB
 
-    for ph in range(depth):
-        fetch(ph)
-    for ph, gfs in enumerate(phases):
-        a(f"  // ---- {', '.join(gfs)}")
-        a("  __syncthreads();")
-        a("#pragma unroll")
-        a("  for (int j = 0; j < NL; j++) {")
-        a("    const int e = tid + NT * j;")
-        a("    if (e < NB) {")
-        for q in range(len(gfs)):
-            a(f"      sh[{q} * NB + e] = pf{(ph % depth) * group + q}[j];")
-        a("    }")
-        a("  }")
-        a("  __syncthreads();")
-        if ph + depth < len(phases):
-            fetch(ph + depth)
+    def emit_phase(ph, gfs):
+        """Derivatives and linear-term contributions of the phase's gridfunctions."""
         for q, g in enumerate(gfs):
             for k in range(P):
-                st = _LdsEmitter(sub, fd_order, f"c{k}", BX, BY, q * NB)
+                st = _LdsEmitter(sub, fd_order, f"c{k}", BX, BY, box(ph, q))
                 # each value is pinned to a register where it is formed: otherwise the
                 # compiler sinks the stencil sums into the guarded algebra below and keeps
                 # every raw LDS value alive across the staging barriers instead
@@ -1169,15 +1191,72 @@ extern "C" __global__ void __launch_bounds__({NT}{f', {min_waves}' if min_waves 
                         tc.append(f"L{k}_coef[{cf['up']}] * (p.invdxx{dirn} * ((L{k}_beta[{dirn}] > 0.0) ? ({su}) : "
                                   f"({sd})))")
                         if k == 0:
-                            flops += 2 * (2 * len(w_up) - 1) + 2
+                            fl[0] += 2 * (2 * len(w_up) - 1) + 2
                     if "ko" in cf:
                         sk = [f"{_dbl(w_ko[0])} * {pts[0]}"]
                         sk += [f"{_dbl(w_ko[o])} * ({pts[o]} + {pts[-o]})" for o in sorted(w_ko) if o > 0]
                         tc.append(f"L{k}_coef[{cf['ko']}] * (p.invdxx{dirn} * ({' + '.join(sk)}))")
                         if k == 0:
-                            flops += 3 * (len(w_ko) // 2) + 3
+                            fl[0] += 3 * (len(w_ko) // 2) + 3
                     a(f"  {acc} += {' + '.join(tc)};")
                     a(f"  BHAH_PIN({acc});")
+
+    if dma:
+        # every lane of every wave issues NJ loads per box (padded boxes), so the number of
+        # loads issued after a phase's own is known at compile time: waiting until at most
+        # that many are outstanding means this phase's boxes have landed (vmcnt counts in
+        # order); the barrier then publishes all waves' boxes
+        def dma_phase(ph):
+            for q, g in enumerate(phases[ph]):
+                a(f"""  {{
+    const double *src = in + {sub.gfs.index(g)} * Ntot;
+#pragma unroll
+    for (int j = 0; j < {NJ}; j++) BHAH_DMA16(src + pofs[j], sh + {box(ph, q)} + 2 * (NT * j + 64 * wv));
+  }}""")
+
+        for ph in range(depth):
+            dma_phase(ph)
+        for ph, gfs in enumerate(phases):
+            later = sum(len(phases[x]) for x in range(ph + 1, min(ph + depth, len(phases)))) * NJ
+            if later > 63:
+                raise ValueError("DMA prefetch deeper than the 6-bit vmcnt counter")
+            a(f"  // ---- {', '.join(gfs)}")
+            a(f"  BHAH_VMWAIT({later});")
+            a("  BHAH_BARRIER();")
+            emit_phase(ph, gfs)
+            if ph + depth < len(phases):
+                a("  BHAH_BARRIER();")  # every wave is done with the ring slot about to be refilled
+                dma_phase(ph + depth)
+    else:
+        for q in range(depth * group):
+            a(f"  double pf{q}[NL];")
+
+        def fetch(ph):
+            for q, g in enumerate(phases[ph]):
+                a(f"""  {{
+    const double *src = in + {sub.gfs.index(g)} * Ntot;
+#pragma unroll
+    for (int j = 0; j < NL; j++) pf{(ph % depth) * group + q}[j] = src[ofs[j]];
+  }}""")
+
+        for ph in range(depth):
+            fetch(ph)
+        for ph, gfs in enumerate(phases):
+            a(f"  // ---- {', '.join(gfs)}")
+            a("  __syncthreads();")
+            a("#pragma unroll")
+            a("  for (int j = 0; j < NL; j++) {")
+            a("    const int e = tid + NT * j;")
+            a("    if (e < NB) {")
+            for q in range(len(gfs)):
+                a(f"      sh[{q} * NB + e] = pf{(ph % depth) * group + q}[j];")
+            a("    }")
+            a("  }")
+            a("  __syncthreads();")
+            if ph + depth < len(phases):
+                fetch(ph + depth)
+            emit_phase(ph, gfs)
+    flops = fl[0]
     for k in range(P):
         a(f"""  {{
     const int i0 = o0 + lx, i1 = o1 + ly, i2 = o2 + lz + {k * ZS};
@@ -1238,7 +1317,8 @@ def candidate_tu(v: specs.Variant, fd_order: int, kind: str, cand: str, ensemble
                                          [f"out[{outs.index(g)} * Ntot + idx]" for g, _ in sub.outputs], xcd=xcd,
                                          depth=next((int(o[1:]) for o in opts if o[0] == "d" and o[1:].isdigit()), 1),
                                          zfirst="zf" in opts,
-                                         group=next((int(o[1:]) for o in opts if o[0] == "g" and o[1:].isdigit()), 1))
+                                         group=next((int(o[1:]) for o in opts if o[0] == "g" and o[1:].isdigit()), 1),
+                                         dma="dma" in opts)
             kernels.append(src)
             calls.append(launch)
             total += fl
@@ -1273,7 +1353,7 @@ extern "C" __global__ void __launch_bounds__({_group_bounds(lb, k, len(groups))}
     consts = "\n".join(f"#define {k} ({v})" for k, v in sorted(rats.items()))
     undefs = "\n".join(f"#undef {k}" for k in sorted(rats))
     if lds:
-        consts = LDS_PIN_MACRO + consts
+        consts = LDS_PIN_MACRO + LDS_DMA_MACROS + consts
     if u32:
         consts = BLD_MACRO + consts
         undefs += "\n#undef BLD\n#undef BHAH_LO\n#undef BHAH_HI"

@@ -9,6 +9,7 @@
 
 #include <cmath>
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -112,6 +113,8 @@ struct bhah_patch {
   bool own_stream = false;
   // selected schedules (indices into k->rhs / k->ricci) and interior tile shape
   int rhs_choice = 0, ricci_choice = 0;
+  // the selected schedules stage through 16-byte global-to-LDS loads (input must be 16-byte aligned)
+  bool rhs_dma = false, ricci_dma = false;
   int tile_x = 64, tile_y = 4;
   // adjoint workspace (allocated on the first MoL_step_adjoint call)
   double *d_adj = nullptr;
@@ -966,7 +969,10 @@ int bhah_hip_rhs_eval(bhah_patch *P, const double *aux, const double *in, double
   if ((rc = check_ptr(aux, "auxevol_gfs")) || (rc = check_ptr(in, "in_gfs")) || (rc = check_ptr(rhs, "rhs_gfs"))) return rc;
   dim3 g, b;
   P->interior_dims(g, b);
-  HIP_TRY(P->k->rhs[P->rhs_choice].launch(g, b, P->stream, in, aux, rhs, P->d_rfm0, P->d_rfm1, P->kp(0, 0)));
+  // a schedule with 16-byte global-to-LDS loads needs a 16-byte aligned input; any other
+  // pointer runs the first (plain) schedule for this call
+  const int c = (P->rhs_dma && ((uintptr_t)in & 15)) ? 0 : P->rhs_choice;
+  HIP_TRY(P->k->rhs[c].launch(g, b, P->stream, in, aux, rhs, P->d_rfm0, P->d_rfm1, P->kp(0, 0)));
   return BHAH_OK;
 }
 
@@ -976,7 +982,8 @@ int bhah_hip_Ricci_eval(bhah_patch *P, const double *in, double *aux) {
   if ((rc = check_ptr(in, "in_gfs")) || (rc = check_ptr(aux, "auxevol_gfs"))) return rc;
   dim3 g, b;
   P->interior_dims(g, b);
-  HIP_TRY(P->k->ricci[P->ricci_choice].launch(g, b, P->stream, in, nullptr, aux, P->d_rfm0, P->d_rfm1, P->kp(0, 0)));
+  const int c = (P->ricci_dma && ((uintptr_t)in & 15)) ? 0 : P->ricci_choice;
+  HIP_TRY(P->k->ricci[c].launch(g, b, P->stream, in, nullptr, aux, P->d_rfm0, P->d_rfm1, P->kp(0, 0)));
   return BHAH_OK;
 }
 
@@ -1185,8 +1192,16 @@ int bhah_hip_patch_select_schedule(bhah_patch *P, int kind, int index) {
   const char *nm = (kind == BHAH_KERNEL_RHS ? P->k->rhs : P->k->ricci)[index].name;
   if (strstr(nm, "_u32") && P->Ntot * 24 * 8 >= (1L << 31))
     return fail(BHAH_ERR_UNSUPPORTED, "schedule %s addresses arrays below 2 GiB only", nm);
-  if (kind == BHAH_KERNEL_RHS) P->rhs_choice = index;
-  else P->ricci_choice = index;
+  const bool dma = strstr(nm, "_dma") != nullptr;
+  if (dma && (P->Nx[0] & 1))
+    return fail(BHAH_ERR_UNSUPPORTED, "schedule %s loads xx0 pairs and needs an even Nx0 (have %d)", nm, P->Nx[0]);
+  if (kind == BHAH_KERNEL_RHS) {
+    P->rhs_choice = index;
+    P->rhs_dma = dma;
+  } else {
+    P->ricci_choice = index;
+    P->ricci_dma = dma;
+  }
   P->invalidate_graph();
   return BHAH_OK;
 }

@@ -82,6 +82,13 @@ struct bhah_cpu_barrier_t {
 };
 inline bhah_cpu_barrier_t bhah_cpu_bar;
 #define __syncthreads() bhah_cpu_bar.wait()
+#define BHAH_BARRIER() bhah_cpu_bar.wait()
+#define BHAH_VMWAIT(n) ((void)0)
+// global-to-LDS load of 16 bytes per lane: lane l writes dst + 16 l (dst is the wave's base)
+#define BHAH_DMA16(src, dst) bhah_cpu_dma16((src), (dst))
+inline void bhah_cpu_dma16(const double *src, double *dst) {
+  __builtin_memcpy(dst + 2 * (threadIdx.x % 64), src, 16);
+}
 
 template <typename K, typename... A>
 void cpu_launch(K kernel, dim3 g, dim3 b, A... args) {
