@@ -924,21 +924,17 @@ def _group_bounds(lb: str, k: int, ngroups: int) -> str:
 
 # Candidate schedules per build: (kind, partition, launch bounds).
 FULL_CANDIDATES = {
-    "rhs": ["p1_lb1", "p2_lb1_sj_lf_pf16_u32_ck6_w2", "p1_lb1_sj_L16x4x4x1_d2_xcd", "p1_lb1_sj_L16x4x4x1_g2_d1_xcd",
-            "p1_lb1_sj_L16x4x4x1_g2_d2_xcd", "p1_lb1_sj_L16x4x4x1_g3_d1_xcd", "p1_lb1_sj_L16x4x4x1_g4_d1_xcd",
-            "p1_lb1_sj_L16x8x2x1_g2_d1_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_xcd", "p1_lb1_sj_L16x4x4x1_d4_dma_xcd",
-            "p1_lb1_sj_L16x4x4x1_d5_dma_xcd", "p1_lb1_sj_L16x8x2x1_d3_dma_xcd"],
-    "ricci": ["r1_lb1", "r2_lb1_sj_lf_pf12_zs16", "r1_lb1_sj_L16x8x2x1_d3_xcd", "r1_lb1_sj_L16x4x4x1_d2_xcd",
-              "r1_lb1_sj_L16x8x2x1_g3_d1_xcd", "r1_lb1_sj_L16x8x2x1_g2_d2_xcd", "r1_lb1_sj_L16x4x4x1_g3_d1_xcd",
-              "r1_lb1_sj_L16x4x4x1_g3_d2_xcd", "r1_lb1_sj_L16x4x4x1_d3_dma_xcd", "r1_lb1_sj_L16x4x4x1_g3_d2_dma_xcd",
-              "r1_lb1_sj_L16x4x4x1_d5_dma_xcd", "r1_lb1_sj_L16x8x2x1_g3_d2_dma_xcd"],
+    "rhs": ["p1_lb1", "p2_lb1_sj_lf_pf16_u32_ck6_w2", "p1_lb1_sj_L16x4x4x1_d2_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_xcd",
+            "p1_lb1_sj_L16x4x4x1_d2_dma_xcd", "p1_lb1_sj_L16x2x8x1_d3_dma_xcd"],
+    "ricci": ["r1_lb1", "r2_lb1_sj_lf_pf12_zs16", "r1_lb1_sj_L16x4x4x1_g3_d2_xcd", "r1_lb1_sj_L16x4x4x1_d3_dma_xcd",
+              "r1_lb1_sj_L16x4x4x1_d2_dma_xcd", "r1_lb1_sj_L16x8x2x1_d3_dma_xcd"],
 }
 SMALL_CANDIDATES = {"rhs": ["p1_lb1", "p1_lb1_sl", "p3_lb1_sl", "p1_lb1_sl_ln_pf16", "p3_lb1_sl_ln_pf16",
                             "p3_lb1_sl_ln_pf16_u32", "p3_lb1_sl_ln_pf16_u32_ck8", "p3_lb1_sl_lf_pf16_u32_ck8",
                             "p1_lb1_sl_lf_pf16_u32", "p1_lb1_sl_lf_pf16_u32_w2"],
                     "ricci": ["r1_lb1", "r1_lb1_sl", "r2_lb1_sl", "r1_lb1_pf16", "r2_lb1_sl_pf16", "r1_lb1_pf16_u32",
                               "r2_lb1_sl_pf16_u32", "r2_lb1_sl_pf16_u32_ck8", "r2_lb1_sl_lf_pf16_u32",
-                              "r2_lb1_sl_lf_pf16_u32_w2", "r2_lb1_sj_L16x4x4x1"]}
+                              "r2_lb1_sl_lf_pf16_u32_w2", "r2_lb1_sj_L16x4x4x1", "r1_lb1_sj_L16x4x4x1_d2_dma_xcd"]}
 
 CAND_SIG = ("dim3 g, dim3 b, hipStream_t s, const double *in, const double *aux, double *out, "
             "const double *rfm0, const double *rfm1, bhah_kparams p")
