@@ -925,9 +925,11 @@ def _group_bounds(lb: str, k: int, ngroups: int) -> str:
 # Candidate schedules per build: (kind, partition, launch bounds).
 FULL_CANDIDATES = {
     "rhs": ["p1_lb1", "p2_lb1_sj_lf_pf16_u32_ck6_w2", "p1_lb1_sj_L16x4x4x1_d2_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_xcd",
-            "p1_lb1_sj_L16x4x4x1_d2_dma_xcd", "p1_lb1_sj_L16x2x8x1_d3_dma_xcd"],
+            "p1_lb1_sj_L16x4x4x1_d2_dma_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_xcd", "p1_lb1_sj_L16x4x4x1_d4_dma_rb1_xcd",
+            "p1_lb1_sj_L16x4x4x1_d2_dma_rb1_xcd", "p1_lb1_sj_L16x4x4x1_g2_d2_dma_rb1_xcd"],
     "ricci": ["r1_lb1", "r2_lb1_sj_lf_pf12_zs16", "r1_lb1_sj_L16x4x4x1_g3_d2_xcd", "r1_lb1_sj_L16x4x4x1_d3_dma_xcd",
-              "r1_lb1_sj_L16x4x4x1_d2_dma_xcd", "r1_lb1_sj_L16x8x2x1_d3_dma_xcd"],
+              "r1_lb1_sj_L16x4x4x1_d2_dma_xcd", "r1_lb1_sj_L16x4x4x1_d3_dma_rb1_xcd", "r1_lb1_sj_L16x4x4x1_d4_dma_rb1_xcd",
+              "r1_lb1_sj_L16x4x4x1_g3_d2_dma_rb1_xcd"],
 }
 SMALL_CANDIDATES = {"rhs": ["p1_lb1", "p1_lb1_sl", "p3_lb1_sl", "p1_lb1_sl_ln_pf16", "p3_lb1_sl_ln_pf16",
                             "p3_lb1_sl_ln_pf16_u32", "p3_lb1_sl_ln_pf16_u32_ck8", "p3_lb1_sl_lf_pf16_u32_ck8",
@@ -1036,7 +1038,8 @@ def _lds_geometry(tok: str) -> Tuple[int, int, int, int]:
 
 def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str], geom: Tuple[int, int, int, int],
                min_waves: int, scope: str, lv: List[str], xcd: bool = False,
-               depth: int = 1, zfirst: bool = False, group: int = 1, dma: bool = False) -> Tuple[str, str, int]:
+               depth: int = 1, zfirst: bool = False, group: int = 1, dma: bool = False,
+               one_barrier: bool = False) -> Tuple[str, str, int]:
     """One workgroup-tiled kernel: each gridfunction with derivatives is staged, one at a
     time, from global memory into a halo box in LDS (the next one's global loads in flight
     while the current one's stencils read LDS; ``group`` gridfunctions per barrier-separated
@@ -1047,7 +1050,8 @@ def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str],
     With ``dma`` the boxes are filled by direct global-to-LDS loads (16 bytes, two xx0
     neighbours, per lane) into a ring of ``depth`` phase buffers, so prefetching takes no
     registers and no LDS store instructions; workgroup barriers are then plain s_barrier
-    with explicit vmcnt waits counted per phase.  The box's xx0 origin is rounded down
+    with explicit vmcnt waits counted per phase (``one_barrier``: one barrier per phase,
+    refilling the slot the previous phase read).  The box's xx0 origin is rounded down
     to an even index (every pair 16-byte aligned), which needs an even Nx0 (checked by the
     host).
 
@@ -1088,6 +1092,9 @@ def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str],
     NL = (NB + NT - 1) // NT
     NJ = (NB // 2 + NT - 1) // NT  # 16-byte DMA loads per thread per box
     SLOT = 2 * NT * NJ if dma else NB  # doubles per box (DMA boxes padded so every lane loads)
+    nslots = (max(1, min(depth, -(-len(staged) // group))) * group) if dma else min(group, len(staged))
+    if 8 * SLOT * nslots > 160 * 1024:
+        raise ValueError(f"{kname}: {8 * SLOT * nslots} bytes of LDS exceed the 160 KiB of a CU")
     # algebra for a placeholder point; each point substitutes its own register names
     ovr: Dict[str, str] = {}
     for n in centre:
@@ -1110,7 +1117,7 @@ extern "C" __global__ void __launch_bounds__({NT}{f', {min_waves}' if min_waves 
         const double *__restrict__ rfm0, const double *__restrict__ rfm1, const bhah_kparams p) {{
   // Tile of {TX}x{TY}x{TZ} interior points; halo box {BX}x{BY}x{BZ} = {NB} values per gridfunction.
   constexpr int BX = {BX}, BY = {BY}, NB = {NB}, NT = {NT}, NL = {NL};
-  __shared__ double sh[{(max(1, min(depth, -(-len(staged) // group))) * group if dma else min(group, len(staged))) * SLOT}];
+  __shared__ double sh[{nslots * SLOT}];
 {_lds_tile_order(xcd, zfirst)}
   const int tid = (int)threadIdx.x;
   const int lx = tid % {TX}, ly = (tid / {TX}) % {TY}, lz = tid / {TX * TY};
@@ -1210,19 +1217,38 @@ extern "C" __global__ void __launch_bounds__({NT}{f', {min_waves}' if min_waves 
     for (int j = 0; j < {NJ}; j++) BHAH_DMA16(src + pofs[j], sh + {box(ph, q)} + 2 * (NT * j + 64 * wv));
   }}""")
 
-        for ph in range(depth):
-            dma_phase(ph)
-        for ph, gfs in enumerate(phases):
-            later = sum(len(phases[x]) for x in range(ph + 1, min(ph + depth, len(phases)))) * NJ
-            if later > 63:
-                raise ValueError("DMA prefetch deeper than the 6-bit vmcnt counter")
-            a(f"  // ---- {', '.join(gfs)}")
-            a(f"  BHAH_VMWAIT({later});")
-            a("  BHAH_BARRIER();")
-            emit_phase(ph, gfs)
-            if ph + depth < len(phases):
-                a("  BHAH_BARRIER();")  # every wave is done with the ring slot about to be refilled
-                dma_phase(ph + depth)
+        if one_barrier:
+            # Refill, right after a phase's barrier, the slot the previous phase read: every
+            # wave passed that barrier only after finishing the previous phase, so one barrier
+            # per phase suffices; loads run depth - 1 phases ahead.
+            if depth < 2:
+                raise ValueError("single-barrier DMA staging needs at least two ring slots")
+            for ph in range(depth - 1):
+                dma_phase(ph)
+            for ph, gfs in enumerate(phases):
+                later = sum(len(phases[x]) for x in range(ph + 1, min(ph + depth - 1, len(phases)))) * NJ
+                if later + group * NJ > 63:
+                    raise ValueError("DMA prefetch deeper than the 6-bit vmcnt counter")
+                a(f"  // ---- {', '.join(gfs)}")
+                a(f"  BHAH_VMWAIT({later});")
+                a("  BHAH_BARRIER();")
+                if ph + depth - 1 < len(phases):
+                    dma_phase(ph + depth - 1)
+                emit_phase(ph, gfs)
+        else:
+            for ph in range(depth):
+                dma_phase(ph)
+            for ph, gfs in enumerate(phases):
+                later = sum(len(phases[x]) for x in range(ph + 1, min(ph + depth, len(phases)))) * NJ
+                if later > 63:
+                    raise ValueError("DMA prefetch deeper than the 6-bit vmcnt counter")
+                a(f"  // ---- {', '.join(gfs)}")
+                a(f"  BHAH_VMWAIT({later});")
+                a("  BHAH_BARRIER();")
+                emit_phase(ph, gfs)
+                if ph + depth < len(phases):
+                    a("  BHAH_BARRIER();")  # every wave is done with the ring slot about to be refilled
+                    dma_phase(ph + depth)
     else:
         for q in range(depth * group):
             a(f"  double pf{q}[NL];")
@@ -1314,7 +1340,7 @@ def candidate_tu(v: specs.Variant, fd_order: int, kind: str, cand: str, ensemble
                                          depth=next((int(o[1:]) for o in opts if o[0] == "d" and o[1:].isdigit()), 1),
                                          zfirst="zf" in opts,
                                          group=next((int(o[1:]) for o in opts if o[0] == "g" and o[1:].isdigit()), 1),
-                                         dma="dma" in opts)
+                                         dma="dma" in opts, one_barrier="rb1" in opts)
             kernels.append(src)
             calls.append(launch)
             total += fl
