@@ -933,10 +933,12 @@ FULL_CANDIDATES = {
 }
 SMALL_CANDIDATES = {"rhs": ["p1_lb1", "p1_lb1_sl", "p3_lb1_sl", "p1_lb1_sl_ln_pf16", "p3_lb1_sl_ln_pf16",
                             "p3_lb1_sl_ln_pf16_u32", "p3_lb1_sl_ln_pf16_u32_ck8", "p3_lb1_sl_lf_pf16_u32_ck8",
-                            "p1_lb1_sl_lf_pf16_u32", "p1_lb1_sl_lf_pf16_u32_w2"],
+                            "p1_lb1_sl_lf_pf16_u32", "p1_lb1_sl_lf_pf16_u32_w2", "p1_lb1_sj_L32x8x1x1_d2_dma",
+                            "p1_lb1_sj_L16x8x2x1_d2_dma", "p1_lb1_sj_L16x8x2x1_d3_dma_rb1"],
                     "ricci": ["r1_lb1", "r1_lb1_sl", "r2_lb1_sl", "r1_lb1_pf16", "r2_lb1_sl_pf16", "r1_lb1_pf16_u32",
                               "r2_lb1_sl_pf16_u32", "r2_lb1_sl_pf16_u32_ck8", "r2_lb1_sl_lf_pf16_u32",
-                              "r2_lb1_sl_lf_pf16_u32_w2", "r2_lb1_sj_L16x4x4x1", "r1_lb1_sj_L16x4x4x1_d2_dma_xcd"]}
+                              "r2_lb1_sl_lf_pf16_u32_w2", "r2_lb1_sj_L16x4x4x1", "r1_lb1_sj_L16x4x4x1_d2_dma_xcd",
+                              "r1_lb1_sj_L32x8x1x1_d2_dma", "r1_lb1_sj_L16x8x2x1_d2_dma", "r1_lb1_sj_L16x8x2x1_d3_dma_rb1"]}
 
 CAND_SIG = ("dim3 g, dim3 b, hipStream_t s, const double *in, const double *aux, double *out, "
             "const double *rfm0, const double *rfm1, bhah_kparams p")
@@ -1039,7 +1041,7 @@ def _lds_geometry(tok: str) -> Tuple[int, int, int, int]:
 def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str], geom: Tuple[int, int, int, int],
                min_waves: int, scope: str, lv: List[str], xcd: bool = False,
                depth: int = 1, zfirst: bool = False, group: int = 1, dma: bool = False,
-               one_barrier: bool = False) -> Tuple[str, str, int]:
+               one_barrier: bool = False, ensemble: str = "") -> Tuple[str, str, int]:
     """One workgroup-tiled kernel: each gridfunction with derivatives is staged, one at a
     time, from global memory into a halo box in LDS (the next one's global loads in flight
     while the current one's stencils read LDS; ``group`` gridfunctions per barrier-separated
@@ -1054,6 +1056,10 @@ def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str],
     refilling the slot the previous phase read).  The box's xx0 origin is rounded down
     to an even index (every pair 16-byte aligned), which needs an even Nx0 (checked by the
     host).
+
+    With ``ensemble`` (a kernel name) the source also holds the batched-ensemble form: the
+    per-tile code as a device function and a kernel walking (member, tile) records, tiles in
+    this kernel's TX x TY x TZ units.
 
     Stencil sums are the same expressions, in the same order, as the global-load kernels.
     Returns (kernel source, launch statement, flops per point)."""
@@ -1111,14 +1117,8 @@ def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str],
     w_ko = dict(fd.weights_1d("dKOD", fd_order))
     L: List[str] = []
     a = L.append
-    a(f"""
-extern "C" __global__ void __launch_bounds__({NT}{f', {min_waves}' if min_waves > 1 else ''})
-{kname}(const double *__restrict__ in, const double *__restrict__ aux, double *__restrict__ out,
-        const double *__restrict__ rfm0, const double *__restrict__ rfm1, const bhah_kparams p) {{
-  // Tile of {TX}x{TY}x{TZ} interior points; halo box {BX}x{BY}x{BZ} = {NB} values per gridfunction.
+    a(f"""  // Tile of {TX}x{TY}x{TZ} interior points; halo box {BX}x{BY}x{BZ} = {NB} values per gridfunction.
   constexpr int BX = {BX}, BY = {BY}, NB = {NB}, NT = {NT}, NL = {NL};
-  __shared__ double sh[{nslots * SLOT}];
-{_lds_tile_order(xcd, zfirst)}
   const int tid = (int)threadIdx.x;
   const int lx = tid % {TX}, ly = (tid / {TX}) % {TY}, lz = tid / {TX * TY};
   const long s1 = p.s1, s2 = p.s2, Ntot = p.Ntot;
@@ -1287,11 +1287,44 @@ extern "C" __global__ void __launch_bounds__({NT}{f', {min_waves}' if min_waves 
 {body.replace("VPT_", f"V{k}_")}
     }}
   }}""")
-    a("}")
+    tile_body = "\n".join(L)
+    bounds = f"{NT}{f', {min_waves}' if min_waves > 1 else ''}"
+    src = f"""
+extern "C" __global__ void __launch_bounds__({bounds})
+{kname}(const double *__restrict__ in, const double *__restrict__ aux, double *__restrict__ out,
+        const double *__restrict__ rfm0, const double *__restrict__ rfm1, const bhah_kparams p) {{
+  __shared__ double sh[{nslots * SLOT}];
+{_lds_tile_order(xcd, zfirst)}
+{tile_body}
+}}
+"""
+    if ensemble:
+        # Batched form: a fixed grid of workgroups walks the ensemble's list of (member, tile)
+        # records; the per-tile code is the same, on the member's arrays.  The barrier before
+        # each tile keeps a fast wave from refilling LDS the previous tile still reads.
+        src += f"""
+static __device__ __forceinline__ void {kname}_tile(const double *__restrict__ in, const double *__restrict__ aux,
+        double *__restrict__ out, const double *__restrict__ rfm0, const double *__restrict__ rfm1,
+        const bhah_kparams p, const unsigned bx, const unsigned by, const unsigned bz, double *sh) {{
+{tile_body}
+}}
+extern "C" __global__ void __launch_bounds__({bounds})
+{ensemble}(const bhah_member *__restrict__ mem, const bhah_item *__restrict__ items, const int nitems,
+        const int in_sel, const int aux_sel, const int out_sel) {{
+  __shared__ double sh[{nslots * SLOT}];
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {{
+    const bhah_item w = items[it];
+    const bhah_member *M = &mem[w.m];
+    {"BHAH_BARRIER();" if dma else "__syncthreads();"}
+    {kname}_tile(M->arr[in_sel], M->arr[aux_sel], M->arr[out_sel], M->rfm0, M->rfm1, M->kp, (unsigned)w.bx,
+                 (unsigned)w.by, (unsigned)w.bz, sh);
+  }}
+}}
+"""
     launch = (f"  hipLaunchKernelGGL({kname}, dim3((p.Nx0 - 2 * p.NG + {TX - 1}) / {TX}, "
               f"(p.Nx1 - 2 * p.NG + {TY - 1}) / {TY}, (p.Nx2 - 2 * p.NG + {TZ - 1}) / {TZ}), dim3({NT}), 0, s, "
               f"in, aux, out, rfm0, rfm1, p);")
-    return "\n".join(L) + "\n", launch, flops
+    return src, launch, flops
 
 
 def candidate_tu(v: specs.Variant, fd_order: int, kind: str, cand: str, ensemble: bool = False) -> Tuple[str, int]:
@@ -1317,8 +1350,6 @@ def candidate_tu(v: specs.Variant, fd_order: int, kind: str, cand: str, ensemble
     zs = next((int(o[2:]) for o in opts if o.startswith("zs") and o[2:].isdigit()), 0)
     wide = "w2" in opts
     lds = next((_lds_geometry(o) for o in opts if o[0] == "L" and o[1:2].isdigit()), None)
-    if lds and ensemble:
-        raise ValueError("LDS-tiled schedules have no batched form")
     rats: Dict[str, str] = {}
     kernels, calls, total = [], [], 0
     ens_kernels, ens_calls = [], []
@@ -1335,15 +1366,19 @@ def candidate_tu(v: specs.Variant, fd_order: int, kind: str, cand: str, ensemble
         kname = f"bssn_{kind}_{tag}_{cand}_g{k}"
         if lds:
             outs = list(bssn.EVOL_GFS if kind == "rhs" else bssn.AUXEVOL_GFS)
+            ename = f"bssn_ens_{kind}_{tag}_{cand}_g{k}" if ensemble else ""
             src, launch, fl = lds_kernel(sub, fd_order, kname, rats, lds, 2 if lb == "lb2" else 1, scope,
                                          [f"out[{outs.index(g)} * Ntot + idx]" for g, _ in sub.outputs], xcd=xcd,
                                          depth=next((int(o[1:]) for o in opts if o[0] == "d" and o[1:].isdigit()), 1),
                                          zfirst="zf" in opts,
                                          group=next((int(o[1:]) for o in opts if o[0] == "g" and o[1:].isdigit()), 1),
-                                         dma="dma" in opts, one_barrier="rb1" in opts)
+                                         dma="dma" in opts, one_barrier="rb1" in opts, ensemble=ename)
             kernels.append(src)
             calls.append(launch)
             total += fl
+            if ensemble:
+                ens_calls.append(f"  hipLaunchKernelGGL({ename}, g, dim3({lds[0] * lds[1] * lds[2] // lds[3]}), 0, s, mem, "
+                                 f"items, nitems, in_sel, aux_sel, out_sel);")
             continue
         if kind == "rhs":
             lv = [f"out[{bssn.EVOL_GFS.index(g)} * Ntot + idx]" for g, _ in sub.outputs]
@@ -1443,21 +1478,26 @@ bssn_rk_det_{tag}(const double *__restrict__ kin, const double *y_n, double *__r
     ens_rk = ""
     if ensemble:
         ens_rk = f"""
+// Records are (member, chunk): 256 consecutive points of the member's arrays, in array order
+// (the single-patch pass's indexing and non-temporal hints).
 static __device__ __forceinline__ void bssn_ens_rk_det_{tag}_tile(const double *__restrict__ kin, const double *y_n,
         double *__restrict__ y_run, double *out, const double *__restrict__ rfm0, const double *__restrict__ rfm1,
-        const bhah_kparams p, const int bx, const int by, const int bz) {{
-{_prologue_at(False)}
+        const bhah_kparams p, const int chunk) {{
+  const long Ntot = p.Ntot;
+  const long idx = (long)chunk * 256 + (threadIdx.x + blockDim.x * threadIdx.y);
+  if (idx >= Ntot) return;
+  const int i0 = (int)(idx % p.Nx0), i1 = (int)((idx / p.Nx0) % p.Nx1);
   constexpr int NGF = {ngf};
   double kk[NGF], ctr[NGF];
 #pragma unroll
   for (int g = 0; g < NGF; g++) {{
-    kk[g] = kin[g * Ntot + idx];
-    ctr[g] = (p.substep == 1) ? y_n[g * Ntot + idx] : 0.0;
+    kk[g] = __builtin_nontemporal_load(&kin[g * Ntot + idx]);
+    ctr[g] = (p.substep == 1) ? __builtin_nontemporal_load(&y_n[g * Ntot + idx]) : 0.0;
   }}
-{RK_EPILOGUE}
+{rk_nt}
 {det}
 #pragma unroll
-  for (int g = 0; g < NGF; g++) out[g * Ntot + idx] = nxt[g];
+  for (int g = 0; g < NGF; g++) __builtin_nontemporal_store(nxt[g], &out[g * Ntot + idx]);
 }}
 extern "C" __global__ void __launch_bounds__(256)
 bssn_ens_rk_det_{tag}(const bhah_member *__restrict__ mem, const bhah_item *__restrict__ items, const int nitems,
@@ -1467,8 +1507,7 @@ bssn_ens_rk_det_{tag}(const bhah_member *__restrict__ mem, const bhah_item *__re
     const bhah_member *M = &mem[w.m];
     bhah_kparams p = M->kp;
     p.substep = substep;
-    bssn_ens_rk_det_{tag}_tile(M->arr[k_sel], M->arr[0], M->arr[1], M->arr[out_sel], M->rfm0, M->rfm1, p, w.bx, w.by,
-                               w.bz);
+    bssn_ens_rk_det_{tag}_tile(M->arr[k_sel], M->arr[0], M->arr[1], M->arr[out_sel], M->rfm0, M->rfm1, p, w.bx);
   }}
 }}
 """
@@ -1615,7 +1654,7 @@ def generate(outdir: str) -> List[str]:
         arrays, ncand = {}, {}
         for kind in ("rhs", "ricci"):
             rows = []
-            kcands = [c for c in cands[kind] if not (ens and is_lds(c))]
+            kcands = list(cands[kind])
             for cand in kcands:
                 src, fl = candidate_tu(v, order, kind, cand, ensemble=ens)
                 path = os.path.join(outdir, f"bssn_{tag}_{kind}_{cand}.hip")
@@ -1624,11 +1663,14 @@ def generate(outdir: str) -> List[str]:
                 paths.append(path)
                 meta["candidates"][f"{kind}_{tag}_{cand}"] = fl
                 table_decl.append(f'extern "C" hipError_t launch_bssn_{kind}_{tag}_{cand}({CAND_SIG});')
+                geo = next((_lds_geometry(o) for o in cand.split("_") if o[:1] == "L" and o[1:2].isdigit()), None)
+                tiles = f"{{{geo[0]}, {geo[1]}, {geo[2]}}}" if geo else "{0, 0, 0}"
                 if ens:
                     table_decl.append(f'extern "C" hipError_t launch_bssn_ens_{kind}_{tag}_{cand}({ENS_SIG});')
-                    rows.append(f'{{"{cand}", launch_bssn_{kind}_{tag}_{cand}, launch_bssn_ens_{kind}_{tag}_{cand}}}')
+                    rows.append(f'{{"{cand}", launch_bssn_{kind}_{tag}_{cand}, launch_bssn_ens_{kind}_{tag}_{cand}, '
+                                f'{tiles}}}')
                 else:
-                    rows.append(f'{{"{cand}", launch_bssn_{kind}_{tag}_{cand}, nullptr}}')
+                    rows.append(f'{{"{cand}", launch_bssn_{kind}_{tag}_{cand}, nullptr, {tiles}}}')
             arrays[kind] = f"kCand_{kind}_{tag}"
             ncand[kind] = len(kcands)
             table_decl.append(f"static const BssnCandidate {arrays[kind]}[] = {{{', '.join(rows)}}};")

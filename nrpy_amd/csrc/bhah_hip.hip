@@ -30,6 +30,7 @@ struct BssnCandidate {
   const char *name;
   bhah_cand_fn launch;
   bhah_ens_fn ens;  // batched-ensemble form, where the build has one
+  int lds[3];       // workgroup tile (points along xx0, xx1, xx2) of LDS-tiled schedules; 0 otherwise
 };
 struct BssnKernels {
   int coord, sym_axes, KO, fd_order;
@@ -305,14 +306,17 @@ ens_bc_radiation_kernel(const bhah_member *__restrict__ mem, const ens_outer_pt 
 __global__ void __launch_bounds__(256)
 ens_bc_inner_kernel(const bhah_member *__restrict__ mem, const ens_inner_pt *__restrict__ pts, int npts, int ngfs,
                     const int *__restrict__ gf_parity, int sel) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)npts * ngfs) return;
-  const int p = (int)(t % npts), gf = (int)(t / npts);
+  // one thread per boundary point, all gridfunctions: the record is read once
+  const int p = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (p >= npts) return;
   const ens_inner_pt e = pts[p];
   const bhah_member *M = &mem[e.m];
-  double *f = &M->arr[sel][gf * M->kp.Ntot];
-  const double v = f[e.src];
-  f[e.dst] = ((e.mask >> gf_parity[gf]) & 1u) ? -v : v;
+  double *f = M->arr[sel];
+  const long Ntot = M->kp.Ntot;
+  for (int gf = 0; gf < ngfs; gf++) {
+    const double v = f[gf * Ntot + e.src];
+    f[gf * Ntot + e.dst] = ((e.mask >> gf_parity[gf]) & 1u) ? -v : v;
+  }
 }
 
 // Quadratic extrapolation f0 = 3 f1 - 3 f2 + f3 along the face normal
@@ -1461,6 +1465,10 @@ struct bhah_ensemble {
   bhah_member *d_mem = nullptr;
   bhah_item *d_int = nullptr, *d_all = nullptr;
   int n_int = 0, n_all = 0;
+  // tile lists of the selected LDS-tiled schedules, in their own tile units ([0] rhs, [1] Ricci)
+  bhah_item *d_lds[2] = {};
+  int n_lds[2] = {};
+  bool aligned16 = true;  // every bound stage array 16-byte aligned (schedules with pair loads need it)
   ens_outer_pt *d_outer[8] = {};
   int n_outer[8] = {};
   ens_inner_pt *d_inner = nullptr;
@@ -1490,7 +1498,8 @@ struct bhah_ensemble {
 namespace {
 
 // Tile lists: interior tiles (tile_x x tile_y threads over (i0, i1), one xx2 plane each)
-// and whole-grid tiles (64 x 4) for the RK pass, all members concatenated.
+// and, for the RK pass, chunks of 256 consecutive points of each member's arrays, all members
+// concatenated.
 int ensemble_build_items(bhah_ensemble *E) {
   std::vector<bhah_item> in, all;
   for (int m = 0; m < (int)E->P.size(); m++) {
@@ -1499,9 +1508,7 @@ int ensemble_build_items(bhah_ensemble *E) {
     for (int bz = 0; bz < n2; bz++)
       for (int by = 0; by < (n1 + E->tile_y - 1) / E->tile_y; by++)
         for (int bx = 0; bx < (n0 + E->tile_x - 1) / E->tile_x; bx++) in.push_back({m, bx, by, bz});
-    for (int bz = 0; bz < P->Nx[2]; bz++)
-      for (int by = 0; by < (P->Nx[1] + 3) / 4; by++)
-        for (int bx = 0; bx < (P->Nx[0] + 63) / 64; bx++) all.push_back({m, bx, by, bz});
+    for (long c = 0; c < (P->Ntot + 255) / 256; c++) all.push_back({m, (int)c, 0, 0});
   }
   hipFree(E->d_int);
   hipFree(E->d_all);
@@ -1513,14 +1520,42 @@ int ensemble_build_items(bhah_ensemble *E) {
   return BHAH_OK;
 }
 
+// Tile list of a selected LDS-tiled schedule (kind 0: rhs, 1: Ricci): every member's interior
+// in the schedule's own TX x TY x TZ tiles.
+int ensemble_build_lds_items(bhah_ensemble *E, int kind) {
+  const BssnCandidate &c = kind == 0 ? E->k->rhs[E->rhs_choice] : E->k->ricci[E->ricci_choice];
+  hipFree(E->d_lds[kind]);
+  E->d_lds[kind] = nullptr;
+  E->n_lds[kind] = 0;
+  if (!c.lds[0]) return BHAH_OK;
+  std::vector<bhah_item> v;
+  for (int m = 0; m < (int)E->P.size(); m++) {
+    const bhah_patch *P = E->P[m];
+    const int t0 = (P->d.Nxx[0] + c.lds[0] - 1) / c.lds[0], t1 = (P->d.Nxx[1] + c.lds[1] - 1) / c.lds[1],
+              t2 = (P->d.Nxx[2] + c.lds[2] - 1) / c.lds[2];
+    for (int bz = 0; bz < t2; bz++)
+      for (int by = 0; by < t1; by++)
+        for (int bx = 0; bx < t0; bx++) v.push_back({m, bx, by, bz});
+  }
+  int rc;
+  if ((rc = upload(v, &E->d_lds[kind]))) return rc;
+  E->n_lds[kind] = (int)v.size();
+  return BHAH_OK;
+}
+
 int ensemble_enqueue_step(bhah_ensemble *E) {
   const int in_s[4] = {0, 2, 3, 2}, k_s[4] = {2, 3, 2, 3};
   const BssnCandidate &ric = E->k->ricci[E->ricci_choice], &rhs = E->k->rhs[E->rhs_choice];
-  const dim3 tb(E->tile_x, E->tile_y, 1), gi = E->grid_for(E->n_int);
+  const dim3 tb(E->tile_x, E->tile_y, 1);
+  if (!E->aligned16 && (strstr(ric.name, "_dma") || strstr(rhs.name, "_dma")))
+    return fail(BHAH_ERR_UNSUPPORTED, "schedules with 16-byte pair loads need 16-byte aligned member arrays");
+  // LDS-tiled schedules walk their own tile lists (the launcher sets the workgroup size)
+  const bhah_item *ric_items = ric.lds[0] ? E->d_lds[1] : E->d_int, *rhs_items = rhs.lds[0] ? E->d_lds[0] : E->d_int;
+  const int ric_n = ric.lds[0] ? E->n_lds[1] : E->n_int, rhs_n = rhs.lds[0] ? E->n_lds[0] : E->n_int;
   for (int s = 1; s <= 4; s++) {
     const int in = in_s[s - 1], k = k_s[s - 1], out = (s == 4) ? 0 : k;
-    HIP_TRY(ric.ens(gi, tb, E->stream, E->d_mem, E->d_int, E->n_int, in, 4, 4));
-    HIP_TRY(rhs.ens(gi, tb, E->stream, E->d_mem, E->d_int, E->n_int, in, 4, k));
+    HIP_TRY(ric.ens(E->grid_for(ric_n), tb, E->stream, E->d_mem, ric_items, ric_n, in, 4, 4));
+    HIP_TRY(rhs.ens(E->grid_for(rhs_n), tb, E->stream, E->d_mem, rhs_items, rhs_n, in, 4, k));
     for (int gz = 0; gz < E->NG; gz++) {
       if (!E->n_outer[gz]) continue;
       dim3 g, b;
@@ -1531,12 +1566,12 @@ int ensemble_enqueue_step(bhah_ensemble *E) {
     }
     if (E->n_inner) {
       dim3 g, b;
-      launch_1d((long)E->n_inner * BHAH_NUM_EVOL_GFS_BSSN, g, b);
+      launch_1d((long)E->n_inner, g, b);
       hipLaunchKernelGGL(ens_bc_inner_kernel, g, b, 0, E->stream, E->d_mem, E->d_inner, E->n_inner,
                          BHAH_NUM_EVOL_GFS_BSSN, E->d_gf_parity, k);
       HIP_TRY(hipGetLastError());
     }
-    HIP_TRY(E->k->ens_rk_det(E->grid_for(E->n_all), dim3(64, 4, 1), E->stream, E->d_mem, E->d_all, E->n_all, k, out, s));
+    HIP_TRY(E->k->ens_rk_det(E->grid_for(E->n_all), dim3(256, 1, 1), E->stream, E->d_mem, E->d_all, E->n_all, k, out, s));
   }
   return BHAH_OK;
 }
@@ -1638,6 +1673,8 @@ int bhah_hip_ensemble_destroy(bhah_ensemble *E) {
   hipFree(E->d_mem);
   hipFree(E->d_int);
   hipFree(E->d_all);
+  hipFree(E->d_lds[0]);
+  hipFree(E->d_lds[1]);
   for (int gz = 0; gz < 8; gz++) hipFree(E->d_outer[gz]);
   hipFree(E->d_inner);
   hipFree(E->d_wavespeed);
@@ -1660,6 +1697,7 @@ int bhah_hip_ensemble_member(bhah_ensemble *E, int m, bhah_patch **patch) {
 
 int bhah_hip_ensemble_bind(bhah_ensemble *E, const bhah_MoL_gridfunctions *gfs, const double *dt) {
   if (!E || !gfs) return fail(BHAH_ERR_ARG, "NULL argument");
+  E->aligned16 = true;
   for (size_t m = 0; m < E->P.size(); m++) {
     const bhah_patch *P = E->P[m];
     const bhah_MoL_gridfunctions &g = gfs[m];
@@ -1675,6 +1713,8 @@ int bhah_hip_ensemble_bind(bhah_ensemble *E, const bhah_MoL_gridfunctions *gfs, 
     M.arr[2] = g.k_odd_gfs;
     M.arr[3] = g.k_even_gfs;
     M.arr[4] = g.auxevol_gfs;
+    for (int a = 0; a < 4; a++)
+      if ((uintptr_t)M.arr[a] & 15) E->aligned16 = false;
     M.coord = P->d.coord;
     M.AMPL = P->d.grid_physical_size;
     M.SINHW = P->d.SINHW;
@@ -1699,13 +1739,16 @@ int bhah_hip_ensemble_select_schedule(bhah_ensemble *E, int kind, int index) {
   if (index < 0 || index >= n) return fail(BHAH_ERR_ARG, "schedule index %d out of range [0,%d)", index, n);
   const BssnCandidate &c = (kind == BHAH_KERNEL_RHS ? E->k->rhs : E->k->ricci)[index];
   if (!c.ens) return fail(BHAH_ERR_UNSUPPORTED, "schedule %s has no batched form", c.name);
-  for (const bhah_patch *P : E->P)
+  for (const bhah_patch *P : E->P) {
     if (strstr(c.name, "_u32") && P->Ntot * 24 * 8 >= (1L << 31))
       return fail(BHAH_ERR_UNSUPPORTED, "schedule %s addresses arrays below 2 GiB only", c.name);
+    if (strstr(c.name, "_dma") && (P->Nx[0] & 1))
+      return fail(BHAH_ERR_UNSUPPORTED, "schedule %s loads xx0 pairs and needs an even Nx0 in every member", c.name);
+  }
   if (kind == BHAH_KERNEL_RHS) E->rhs_choice = index;
   else E->ricci_choice = index;
   E->invalidate_graph();
-  return BHAH_OK;
+  return ensemble_build_lds_items(E, kind == BHAH_KERNEL_RHS ? 0 : 1);
 }
 
 int bhah_hip_ensemble_set_launch(bhah_ensemble *E, int tile_x, int tile_y, int max_workgroups) {

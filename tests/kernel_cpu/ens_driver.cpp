@@ -2,7 +2,8 @@
  * CPU over several members of different grid sizes.  Built with
  * -DKERNEL_FILE="<gen .hip>" -DLAUNCH=<ensemble launcher> -DKIND_RHS=0|1.
  *
- * Input: int32 nmem tile_x tile_y max_wg; per member: int32 Nx0 Nx1 Nx2 NG,
+ * Input: int32 nmem tile_x tile_y max_wg tile_z (interior tiles of tile_x x tile_y x tile_z
+ * points: tile_z > 1 for LDS-tiled schedules); per member: int32 Nx0 Nx1 Nx2 NG,
  * float64 invdxx0..2 eta KO_gauge KO_nongauge wavespeed, in[24*Ntot],
  * aux[6*Ntot] (rhs only), rfm0[4*Nx0], rfm1[3*Nx1].  Output: per member
  * out[nout*Ntot].  Every array is its own exact-size heap block. */
@@ -21,9 +22,9 @@ static double *rd(FILE *f, size_t n) {
 int main(int argc, char **argv) {
   if (argc != 3) return 2;
   FILE *f = fopen(argv[1], "rb");
-  int h[4];
-  if (!f || fread(h, sizeof(int), 4, f) != 4) return 2;
-  const int n = h[0], tx = h[1], ty = h[2], maxwg = h[3];
+  int h[5];
+  if (!f || fread(h, sizeof(int), 5, f) != 5) return 2;
+  const int n = h[0], tx = h[1], ty = h[2], maxwg = h[3], tz = h[4];
   std::vector<bhah_member> mem(n);
   std::vector<bhah_item> items;
   std::vector<size_t> nout(n);
@@ -49,7 +50,7 @@ int main(int argc, char **argv) {
     mem[m].arr[2] = (double *)calloc(nout[m], sizeof(double));
     if (!KIND_RHS) mem[m].arr[4] = mem[m].arr[2];
     const int n0 = p.Nx0 - 2 * p.NG, n1 = p.Nx1 - 2 * p.NG, n2 = p.Nx2 - 2 * p.NG;
-    for (int bz = 0; bz < n2; bz++)
+    for (int bz = 0; bz < (n2 + tz - 1) / tz; bz++)
       for (int by = 0; by < (n1 + ty - 1) / ty; by++)
         for (int bx = 0; bx < (n0 + tx - 1) / tx; bx++) items.push_back({m, bx, by, bz});
   }

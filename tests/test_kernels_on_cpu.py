@@ -118,10 +118,14 @@ ENS_DRIVER = os.path.join(HERE, "kernel_cpu", "ens_driver.cpp")
 
 
 @pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
-@pytest.mark.parametrize("kind,cand", [("ricci", "r2_lb1_sl_pf16"), ("rhs", "p3_lb1_sl_ln_pf16")])
-def test_ensemble_kernels_on_cpu(oracle_lib, kind, cand):
+@pytest.mark.parametrize("kind,cand,tile", [("ricci", "r2_lb1_sl_pf16", (64, 4, 1)),
+                                            ("rhs", "p3_lb1_sl_ln_pf16", (64, 4, 1)),
+                                            ("ricci", "r1_lb1_sj_L16x8x2x1_d2_dma", (16, 8, 2)),
+                                            ("rhs", "p1_lb1_sj_L16x8x2x1_d3_dma_rb1", (16, 8, 2))])
+def test_ensemble_kernels_on_cpu(oracle_lib, kind, cand, tile):
     """Batched-ensemble form of a schedule over members of different grids, with a
-    persistent grid smaller than the tile list, matches the oracle per member."""
+    persistent grid smaller than the tile list, matches the oracle per member (LDS-tiled
+    schedules walk tiles in their own units, each workgroup's threads run concurrently)."""
     tag = "Spherical_sym2_KO0_fd4"
     grids = [(12, 8, 2), (16, 6, 4), (10, 4, 2)]
     Os, ys, auxs, refs = [], [], [], []
@@ -137,13 +141,14 @@ def test_ensemble_kernels_on_cpu(oracle_lib, kind, cand):
     try:
         src = os.path.join(GEN, f"bssn_{tag}_{kind}_{cand}.hip")
         exe = os.path.join(wd, "ens")
+        threaded = ["-DBHAH_CPU_THREADED=1", "-pthread"] if tile[2] > 1 else []
         subprocess.run(["g++", "-std=c++17", "-O1", "-g", "-fsanitize=address", "-fno-omit-frame-pointer", "-w",
-                        f"-I{SHIM}", f'-DKERNEL_FILE="{src}"', f"-DLAUNCH=launch_bssn_ens_{kind}_{tag}_{cand}",
+                        *threaded, f"-I{SHIM}", f'-DKERNEL_FILE="{src}"', f"-DLAUNCH=launch_bssn_ens_{kind}_{tag}_{cand}",
                         f"-DKIND_RHS={int(kind == 'rhs')}", ENS_DRIVER, "-o", exe, "-lm"],
                        check=True, capture_output=True)
         inp, outp = os.path.join(wd, "in.bin"), os.path.join(wd, "out.bin")
         with open(inp, "wb") as f:
-            np.array([len(grids), 64, 4, 3], dtype=np.int32).tofile(f)
+            np.array([len(grids), tile[0], tile[1], 3, tile[2]], dtype=np.int32).tofile(f)
             for O, y, aux in zip(Os, ys, auxs):
                 np.array([*O.Nx, O.NG], dtype=np.int32).tofile(f)
                 np.array([1.0 / (O.xx[d][1] - O.xx[d][0]) for d in range(3)] + [2.0, 0.3, 0.3, 1.0]).tofile(f)
