@@ -1042,12 +1042,11 @@ def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str],
                min_waves: int, scope: str, lv: List[str], xcd: bool = False,
                depth: int = 1, zfirst: bool = False, group: int = 1, dma: bool = False,
                one_barrier: bool = False, ensemble: str = "") -> Tuple[str, str, int]:
-    """One workgroup-tiled kernel: each gridfunction with derivatives is staged, one at a
-    time, from global memory into a halo box in LDS (the next one's global loads in flight
-    while the current one's stencils read LDS; ``group`` gridfunctions per barrier-separated
-    phase, ``depth`` phases ahead), every
-    derivative of the thread's points is
-    formed from the box, then the point algebra runs with all inputs in registers.
+    """One workgroup-tiled kernel.  Each gridfunction with derivatives is staged, one at a
+    time, from global memory into a halo box in LDS: ``group`` gridfunctions per
+    barrier-separated phase, with the loads of the next ``depth`` phases in flight while
+    the current box is read.  Every derivative of the thread's points is formed from the
+    box, then the point algebra runs with all inputs in registers.
 
     With ``dma`` the boxes are filled by direct global-to-LDS loads (16 bytes, two xx0
     neighbours, per lane) into a ring of ``depth`` phase buffers, so prefetching takes no
