@@ -142,6 +142,44 @@ def test_mol_steps_match_oracle(hip, oracle_lib, case, use_graph):
     assert er.max() < 1e-10, er
 
 
+def test_pair_load_schedules_guard_odd_rows_and_unaligned_input(hip, oracle_lib):
+    """Schedules that copy 16-byte xx0 pairs into LDS are refused on grids with an odd Nx0,
+    and a call whose input is not 16-byte aligned runs the build's first schedule instead
+    (same values as the oracle)."""
+    import torch
+    from nrpy_amd.runtime import KERNEL_RHS, KERNEL_RICCI, BHaHError, Patch, PatchConfig
+
+    c = CASES["sinhspherical_fd6"]
+    odd = PatchConfig(coord=c["coord"], Nxx=(21, 10, 12), fd_order=6, symmetry_axes=0, enable_KO=True,
+                      grid_physical_size=c["size"], SINHW=c["sinhw"], eta=2.0)
+    Podd = Patch(odd)
+    dma = [n for n in Podd.schedules(KERNEL_RHS) + Podd.schedules(KERNEL_RICCI) if "_dma" in n]
+    assert dma
+    for name in dma:
+        kind = KERNEL_RHS if name.startswith("p") else KERNEL_RICCI
+        with pytest.raises(BHaHError, match="even Nx0"):
+            Podd.select_schedule(kind, name)
+
+    P, O = make_pair(hip, oracle_lib, "sinhspherical_fd6")
+    y = state(O, 9)
+    aux_o = O.ricci(y)
+    rhs_o = O.rhs(y, aux_o, O.params())
+    NG = O.NG
+    flat = torch.zeros(y.size + 1, dtype=torch.float64, device="cuda")
+    yd = flat[1:].view(y.shape)  # 8 bytes past a 16-byte boundary
+    yd.copy_(torch.from_numpy(y))
+    assert yd.data_ptr() % 16 == 8
+    P.select_schedule(KERNEL_RHS, next(n for n in P.schedules(KERNEL_RHS) if "_dma" in n))
+    P.select_schedule(KERNEL_RICCI, next(n for n in P.schedules(KERNEL_RICCI) if "_dma" in n))
+    a = P.new_gfs(6)
+    P.Ricci_eval(yd, a)
+    r = P.new_gfs()
+    P.rhs_eval(torch.from_numpy(aux_o).cuda(), yd, r)
+    torch.cuda.synchronize()
+    assert rel_err(interior(a.cpu().numpy(), NG), interior(aux_o, NG)).max() < 1e-12
+    assert rel_err(interior(r.cpu().numpy(), NG), interior(rhs_o, NG)).max() < 1e-12
+
+
 @pytest.mark.parametrize("case", list(CASES))
 def test_constraints_match_oracle(hip, oracle_lib, case):
     import torch
