@@ -926,10 +926,12 @@ def _group_bounds(lb: str, k: int, ngroups: int) -> str:
 FULL_CANDIDATES = {
     "rhs": ["p1_lb1", "p2_lb1_sj_lf_pf16_u32_ck6_w2", "p1_lb1_sj_L16x4x4x1_d2_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_xcd",
             "p1_lb1_sj_L16x4x4x1_d2_dma_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_xcd", "p1_lb1_sj_L16x4x4x1_d4_dma_rb1_xcd",
-            "p1_lb1_sj_L16x4x4x1_d2_dma_rb1_xcd", "p1_lb1_sj_L16x4x4x1_g2_d2_dma_rb1_xcd"],
+            "p1_lb1_sj_L16x4x4x1_d2_dma_rb1_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_xcd",
+            "p1_lb1_sj_L16x4x4x1_d4_dma_rb1_io_xcd"],
     "ricci": ["r1_lb1", "r2_lb1_sj_lf_pf12_zs16", "r1_lb1_sj_L16x4x4x1_g3_d2_xcd", "r1_lb1_sj_L16x4x4x1_d3_dma_xcd",
               "r1_lb1_sj_L16x4x4x1_d2_dma_xcd", "r1_lb1_sj_L16x4x4x1_d3_dma_rb1_xcd", "r1_lb1_sj_L16x4x4x1_d4_dma_rb1_xcd",
-              "r1_lb1_sj_L16x4x4x1_g3_d2_dma_rb1_xcd"],
+              "r1_lb1_sj_L16x4x4x1_g3_d2_dma_rb1_xcd", "r1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_xcd",
+              "r1_lb1_sj_L16x4x4x1_d2_dma_io_xcd"],
 }
 SMALL_CANDIDATES = {"rhs": ["p1_lb1", "p1_lb1_sl", "p3_lb1_sl", "p1_lb1_sl_ln_pf16", "p3_lb1_sl_ln_pf16",
                             "p3_lb1_sl_ln_pf16_u32", "p3_lb1_sl_ln_pf16_u32_ck8", "p3_lb1_sl_lf_pf16_u32_ck8",
@@ -1041,7 +1043,7 @@ def _lds_geometry(tok: str) -> Tuple[int, int, int, int]:
 def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str], geom: Tuple[int, int, int, int],
                min_waves: int, scope: str, lv: List[str], xcd: bool = False,
                depth: int = 1, zfirst: bool = False, group: int = 1, dma: bool = False,
-               one_barrier: bool = False, ensemble: str = "") -> Tuple[str, str, int]:
+               one_barrier: bool = False, ensemble: str = "", interleave: bool = False) -> Tuple[str, str, int]:
     """One workgroup-tiled kernel.  Each gridfunction with derivatives is staged, one at a
     time, from global memory into a halo box in LDS: ``group`` gridfunctions per
     barrier-separated phase, with the loads of the next ``depth`` phases in flight while
@@ -1088,6 +1090,18 @@ def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str],
             line_out[key] = g_out
     H = fd.ghost_zones(fd_order) if terms else fd_order // 2
     staged = [g for g in sub.gfs if any(d.gf == g for d in dused) or any(k[0] == g for k in line_out)]
+    if interleave:
+        # spread the gridfunctions with mixed second derivatives (several times the LDS reads
+        # and arithmetic of the others) evenly over the phases, so that the loads running
+        # ahead meet phases of similar length
+        heavy = [g for g in staged if any(d.gf == g and d.op == "dDD" and d.dirs[0] != d.dirs[1] for d in dused)]
+        light = [g for g in staged if g not in heavy]
+        order: List[str] = []
+        for i, g in enumerate(heavy):
+            lo = (i * len(light)) // max(1, len(heavy))
+            hi = ((i + 1) * len(light)) // max(1, len(heavy))
+            order += [g] + light[lo:hi]
+        staged = order if heavy else staged
     # xx0 shift of the box origin that makes it even (DMA pairs): NG - H is fixed per build
     SH = ((fd.ghost_zones(fd_order) - H) & 1) if dma else 0
     if dma and TX % 2:
@@ -1371,7 +1385,8 @@ def candidate_tu(v: specs.Variant, fd_order: int, kind: str, cand: str, ensemble
                                          depth=next((int(o[1:]) for o in opts if o[0] == "d" and o[1:].isdigit()), 1),
                                          zfirst="zf" in opts,
                                          group=next((int(o[1:]) for o in opts if o[0] == "g" and o[1:].isdigit()), 1),
-                                         dma="dma" in opts, one_barrier="rb1" in opts, ensemble=ename)
+                                         dma="dma" in opts, one_barrier="rb1" in opts, ensemble=ename,
+                                         interleave="io" in opts)
             kernels.append(src)
             calls.append(launch)
             total += fl
