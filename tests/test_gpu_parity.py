@@ -142,6 +142,62 @@ def test_mol_steps_match_oracle(hip, oracle_lib, case, use_graph):
     assert er.max() < 1e-10, er
 
 
+def test_head_on_collision_over_many_steps(hip, oracle_lib):
+    """The Spherical 72x12x2 head-on Brill-Lindquist evolution (fd4, radiation BCs, eta = 2)
+    over 300 RK4 steps with hipGraph-captured steps: fields and the Hamiltonian-constraint
+    field stay with the oracle's to 1e-9 of their scale."""
+    import torch
+    from nrpy_amd.runtime import MoLState, Patch, PatchConfig
+
+    cfg = PatchConfig(coord=1, Nxx=(72, 12, 2), fd_order=4, symmetry_axes=4, enable_KO=False,
+                      grid_physical_size=7.5, SINHW=0.2, eta=2.0)
+    P = Patch(cfg)
+    O = oracle_lib.OracleGrid(1, (72, 12, 2), 4, 7.5, 0.2, variant=0)
+    y = ID.brill_lindquist(O.coord, O.xx, O.g.p0, O.g.p1)
+    O.apply_bcs_outerextrap_and_inner(y)
+    y = np.ascontiguousarray(y)
+    st_o = O.new_state(y)
+    S = MoLState(P, torch.from_numpy(y.copy()).cuda())
+    dt = P.dt
+    nsteps = 300
+    for _ in range(nsteps):
+        O.bssn_mol_step(st_o, O.params(), dt)
+    S.steps(nsteps, dt, use_graph=True)
+    torch.cuda.synchronize()
+    yd = S.y_n.cpu().numpy()
+    assert rel_err(yd, st_o["y_n"]).max() < 1e-9
+    NG = O.NG
+    aux_o = O.ricci(st_o["y_n"])
+    c_o = O.constraints(st_o["y_n"], aux_o)
+    aux_d = P.new_gfs(6)
+    P.Ricci_eval(S.y_n, aux_d)
+    diag = P.new_gfs(5)
+    P.diagnostics_constraints(aux_d, S.y_n, diag)
+    er = rel_err(interior(diag.cpu().numpy(), NG)[:1], interior(c_o, NG)[:1])
+    assert er.max() < 1e-9, er
+
+
+def test_sinhspherical_fd6_over_many_steps(hip, oracle_lib):
+    """The 6th-order + KO SinhSpherical build with its tuned (workgroup-tiled) schedules over
+    120 hipGraph-captured RK4 steps of perturbed Brill-Lindquist data: fields stay with the
+    oracle's to 1e-9 of their scale."""
+    import torch
+    from nrpy_amd.runtime import KERNEL_RHS, KERNEL_RICCI, MoLState
+
+    P, O = make_pair(hip, oracle_lib, "sinhspherical_fd6")
+    assert "_L" in P.schedule(KERNEL_RHS) and "_L" in P.schedule(KERNEL_RICCI)
+    y = state(O, 17)
+    st_o = O.new_state(y)
+    S = MoLState(P, torch.from_numpy(y.copy()).cuda())
+    dt = P.dt
+    nsteps = 120
+    for _ in range(nsteps):
+        O.bssn_mol_step(st_o, O.params(), dt)
+    S.steps(nsteps, dt, use_graph=True)
+    torch.cuda.synchronize()
+    assert rel_err(S.y_n.cpu().numpy(), st_o["y_n"]).max() < 1e-9
+
+
 def test_pair_load_schedules_guard_odd_rows_and_unaligned_input(hip, oracle_lib):
     """Schedules that copy 16-byte xx0 pairs into LDS are refused on grids with an odd Nx0,
     and a call whose input is not 16-byte aligned runs the build's first schedule instead
