@@ -993,6 +993,9 @@ def _lds_tile_order(xcd: bool, zfirst: bool) -> str:
 LDS_DMA_MACROS = """#ifndef BHAH_DMA16
 // 16 bytes per lane straight from global memory into LDS at dst + 16 * lane (dst uniform per wave)
 #define BHAH_DMA16(src, dst) __builtin_amdgcn_global_load_lds((const void *)(src), (__attribute__((address_space(3))) void *)(dst), 16, 0, 0)
+// the same from a uniform byte base plus a 32-bit per-lane byte offset, widened at the load
+// itself so it is addressed as scalar base + vector offset
+#define BHAH_DMA16O(base, off, dst) do { unsigned o_ = (off); asm volatile("" : "+v"(o_)); BHAH_DMA16((base) + o_, dst); } while (0)
 // wait until at most n vector-memory operations of this wave are outstanding (gfx9 encoding)
 #define BHAH_VMWAIT(n) __builtin_amdgcn_s_waitcnt(((n) & 15) | (((n) >> 4) << 14) | 0x0F70)
 // workgroup barrier without a memory fence (LDS reads of this wave retired first)
@@ -1015,15 +1018,48 @@ def _lds_offsets(dma: bool, H: int, SH: int, NJ: int, NL: int) -> str:
   }}"""
     # pair j of this thread: box elements 2q, 2q + 1 (q = tid + NT j) are xx0 neighbours
     # (BX is even); a pair start past Nx0 - 2 only feeds points outside the interior
-    return f"""  const int wv = tid >> 6;
-  int pofs[{NJ}];
+    # The wave index is made uniform so the LDS destination of every load is scalar
+    # arithmetic, and the offsets are 32-bit byte offsets from a uniform gridfunction base:
+    # each load is then one scalar-base + vector-offset instruction (the host keeps a
+    # gridfunction under 4 GiB for these schedules).
+    return f"""  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  unsigned pofs[{NJ}];
 #pragma unroll
   for (int j = 0; j < {NJ}; j++) {{
     const int e = 2 * (tid + NT * j);
     const int ex = e % BX, ey = (e / BX) % BY, ez = e / (BX * BY);
     const int g0 = min(o0 - {H + SH} + ex, p.Nx0 - 2), g1 = min(o1 - {H} + ey, p.Nx1 - 1), g2 = min(o2 - {H} + ez, p.Nx2 - 1);
-    pofs[j] = g0 + (int)s1 * g1 + (int)s2 * g2;
+    pofs[j] = 8u * (unsigned)(g0 + (int)s1 * g1 + (int)s2 * g2);
   }}"""
+
+
+def _int_sum(w: Dict[int, sp.Rational], D: int, pts: Dict[int, str]) -> str:
+    """D * sum_o w[o] pts[o] as a C expression with integer weights, led by a weight of
+    magnitude one where the stencil has one (a plain value starts the multiply-add chain),
+    the overall sign taken out."""
+    W = {o: sp.Rational(v) * D for o, v in w.items()}
+    order = sorted(W, key=lambda o: (abs(W[o]) != 1, -abs(o)))
+    sgn = -1 if W[order[0]] < 0 else 1
+    parts = []
+    for o in order:
+        c = int(sgn * W[o])
+        parts.append(pts[o] if (c == 1 and not parts) else f"{c}.0 * {pts[o]}")
+    return f"{'-' if sgn < 0 else ''}({' + '.join(parts)})"
+
+
+def _int_pair_sum(w: Dict[int, sp.Rational], D: int, pts: Dict[int, str]) -> str:
+    """D * sum_o w[o] pts[o] for a stencil symmetric about its centre, with integer weights
+    on the sums of mirrored points, outermost pair first, the overall sign taken out so
+    that a weight of magnitude one leads."""
+    W = {o: sp.Rational(v) * D for o, v in w.items() if o >= 0}
+    order = sorted(W, reverse=True)
+    sgn = -1 if W[order[0]] < 0 else 1
+    parts = []
+    for o in order:
+        c = int(sgn * W[o])
+        v = f"({pts[o]} + {pts[-o]})" if o else pts[0]
+        parts.append(v if (c == 1 and not parts) else f"{c}.0 * {v}")
+    return f"{'-' if sgn < 0 else ''}({' + '.join(parts)})"
 
 
 def is_lds(cand: str) -> bool:
@@ -1139,6 +1175,13 @@ def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str],
 {_lds_offsets(dma, H, SH, NJ, NL)}""")
     for k in range(P):
         a(f"  const int c{k} = (lx + {H + SH}) + BX * (ly + {H}) + BX * BY * (lz + {k * ZS + H});")
+    # Upwind sums are formed with integer weights (their common denominator DU moved into
+    # one uniform scale per direction): an FP64 multiply-add takes an integer weight as an
+    # instruction literal, where a weight such as 1/30 needs a register pair loaded first.
+    DU = int(sp.ilcm(*[sp.Rational(v).q for v in list(w_up.values()) + list(w_dn.values())]))
+    DK = int(sp.ilcm(*[sp.Rational(v).q for v in w_ko.values()]))
+    for d in range(3):
+        a(f"  const double ius{d} = p.invdxx{d} * (1.0 / {DU}.0), ikos{d} = p.invdxx{d} * (1.0 / {DK}.0);")
     if terms:
         # coefficients of the linear terms and the upwind control vector at each point
         crep, cred = sp.cse(coefs, symbols=sp.numbered_symbols("lc"), order="none")
@@ -1202,16 +1245,12 @@ def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str],
                     pts = {o: st.load(g, tuple(o if r == dirn else 0 for r in range(3))) for o in range(-H, H + 1)}
                     tc = []
                     if "up" in cf:
-                        su = " + ".join(f"{_dbl(w)} * {pts[o]}" for o, w in sorted(w_up.items()))
-                        sd = " + ".join(f"{_dbl(w)} * {pts[o]}" for o, w in sorted(w_dn.items()))
-                        tc.append(f"L{k}_coef[{cf['up']}] * (p.invdxx{dirn} * ((L{k}_beta[{dirn}] > 0.0) ? ({su}) : "
-                                  f"({sd})))")
+                        su, sd = _int_sum(w_up, DU, pts), _int_sum(w_dn, DU, pts)
+                        tc.append(f"L{k}_coef[{cf['up']}] * (ius{dirn} * ((L{k}_beta[{dirn}] > 0.0) ? {su} : {sd}))")
                         if k == 0:
                             fl[0] += 2 * (2 * len(w_up) - 1) + 2
                     if "ko" in cf:
-                        sk = [f"{_dbl(w_ko[0])} * {pts[0]}"]
-                        sk += [f"{_dbl(w_ko[o])} * ({pts[o]} + {pts[-o]})" for o in sorted(w_ko) if o > 0]
-                        tc.append(f"L{k}_coef[{cf['ko']}] * (p.invdxx{dirn} * ({' + '.join(sk)}))")
+                        tc.append(f"L{k}_coef[{cf['ko']}] * (ikos{dirn} * {_int_pair_sum(w_ko, DK, pts)})")
                         if k == 0:
                             fl[0] += 3 * (len(w_ko) // 2) + 3
                     a(f"  {acc} += {' + '.join(tc)};")
@@ -1225,9 +1264,9 @@ def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str],
         def dma_phase(ph):
             for q, g in enumerate(phases[ph]):
                 a(f"""  {{
-    const double *src = in + {sub.gfs.index(g)} * Ntot;
+    const char *src = (const char *)(in + {sub.gfs.index(g)} * Ntot);
 #pragma unroll
-    for (int j = 0; j < {NJ}; j++) BHAH_DMA16(src + pofs[j], sh + {box(ph, q)} + 2 * (NT * j + 64 * wv));
+    for (int j = 0; j < {NJ}; j++) BHAH_DMA16O(src, pofs[j], sh + {box(ph, q)} + 2 * (NT * j + 64 * wv));
   }}""")
 
         if one_barrier:

@@ -1199,6 +1199,8 @@ int bhah_hip_patch_select_schedule(bhah_patch *P, int kind, int index) {
   const bool dma = strstr(nm, "_dma") != nullptr;
   if (dma && (P->Nx[0] & 1))
     return fail(BHAH_ERR_UNSUPPORTED, "schedule %s loads xx0 pairs and needs an even Nx0 (have %d)", nm, P->Nx[0]);
+  if (dma && P->Ntot * 8 >= (1L << 32))
+    return fail(BHAH_ERR_UNSUPPORTED, "schedule %s addresses a gridfunction below 4 GiB only", nm);
   if (kind == BHAH_KERNEL_RHS) {
     P->rhs_choice = index;
     P->rhs_dma = dma;
@@ -1744,6 +1746,8 @@ int bhah_hip_ensemble_select_schedule(bhah_ensemble *E, int kind, int index) {
       return fail(BHAH_ERR_UNSUPPORTED, "schedule %s addresses arrays below 2 GiB only", c.name);
     if (strstr(c.name, "_dma") && (P->Nx[0] & 1))
       return fail(BHAH_ERR_UNSUPPORTED, "schedule %s loads xx0 pairs and needs an even Nx0 in every member", c.name);
+    if (strstr(c.name, "_dma") && P->Ntot * 8 >= (1L << 32))
+      return fail(BHAH_ERR_UNSUPPORTED, "schedule %s addresses a gridfunction below 4 GiB only", c.name);
   }
   if (kind == BHAH_KERNEL_RHS) E->rhs_choice = index;
   else E->ricci_choice = index;

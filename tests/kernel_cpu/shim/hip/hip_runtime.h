@@ -84,9 +84,12 @@ inline bhah_cpu_barrier_t bhah_cpu_bar;
 #define __syncthreads() bhah_cpu_bar.wait()
 #define BHAH_BARRIER() bhah_cpu_bar.wait()
 #define BHAH_VMWAIT(n) ((void)0)
+// every lane of a wave holds the same value where the kernels use it (the wave index)
+#define __builtin_amdgcn_readfirstlane(x) (x)
 // global-to-LDS load of 16 bytes per lane: lane l writes dst + 16 l (dst is the wave's base)
 #define BHAH_DMA16(src, dst) bhah_cpu_dma16((src), (dst))
-inline void bhah_cpu_dma16(const double *src, double *dst) {
+#define BHAH_DMA16O(base, off, dst) bhah_cpu_dma16((base) + (off), (dst))
+inline void bhah_cpu_dma16(const void *src, double *dst) {
   __builtin_memcpy(dst + 2 * (threadIdx.x % 64), src, 16);
 }
 
