@@ -1033,10 +1033,10 @@ def _lds_offsets(dma: bool, H: int, SH: int, NJ: int, NL: int) -> str:
   }}"""
 
 
-def _int_sum(w: Dict[int, sp.Rational], D: int, pts: Dict[int, str]) -> str:
-    """D * sum_o w[o] pts[o] as a C expression with integer weights, led by a weight of
-    magnitude one where the stencil has one (a plain value starts the multiply-add chain),
-    the overall sign taken out."""
+def _int_sum(w: Dict[int, sp.Rational], D: int, pts: Dict[int, str]) -> Tuple[int, str]:
+    """(sign, S) with D * sum_o w[o] pts[o] = sign * S and S a C expression with integer
+    weights, led by a weight of one where the stencil has a weight of magnitude one (a
+    plain value starts the multiply-add chain)."""
     W = {o: sp.Rational(v) * D for o, v in w.items()}
     order = sorted(W, key=lambda o: (abs(W[o]) != 1, -abs(o)))
     sgn = -1 if W[order[0]] < 0 else 1
@@ -1044,7 +1044,7 @@ def _int_sum(w: Dict[int, sp.Rational], D: int, pts: Dict[int, str]) -> str:
     for o in order:
         c = int(sgn * W[o])
         parts.append(pts[o] if (c == 1 and not parts) else f"{c}.0 * {pts[o]}")
-    return f"{'-' if sgn < 0 else ''}({' + '.join(parts)})"
+    return sgn, f"({' + '.join(parts)})"
 
 
 def _int_pair_sum(w: Dict[int, sp.Rational], D: int, pts: Dict[int, str]) -> str:
@@ -1098,7 +1098,8 @@ def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str],
     per-tile code as a device function and a kernel walking (member, tile) records, tiles in
     this kernel's TX x TY x TZ units.
 
-    Stencil sums are the same expressions, in the same order, as the global-load kernels.
+    Centred stencil sums are the same expressions, in the same order, as the global-load
+    kernels; upwind and KO sums use integer weights with the denominator in a uniform scale.
     Returns (kernel source, launch statement, flops per point)."""
     TX, TY, TZ, P = geom
     NT = TX * TY * TZ // P
@@ -1181,7 +1182,8 @@ def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str],
     DU = int(sp.ilcm(*[sp.Rational(v).q for v in list(w_up.values()) + list(w_dn.values())]))
     DK = int(sp.ilcm(*[sp.Rational(v).q for v in w_ko.values()]))
     for d in range(3):
-        a(f"  const double ius{d} = p.invdxx{d} * (1.0 / {DU}.0), ikos{d} = p.invdxx{d} * (1.0 / {DK}.0);")
+        a(f"  const double ius{d} = p.invdxx{d} * (1.0 / {DU}.0), nius{d} = -ius{d}, "
+          f"ikos{d} = p.invdxx{d} * (1.0 / {DK}.0);")
     if terms:
         # coefficients of the linear terms and the upwind control vector at each point
         crep, cred = sp.cse(coefs, symbols=sp.numbered_symbols("lc"), order="none")
@@ -1245,8 +1247,11 @@ def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str],
                     pts = {o: st.load(g, tuple(o if r == dirn else 0 for r in range(3))) for o in range(-H, H + 1)}
                     tc = []
                     if "up" in cf:
-                        su, sd = _int_sum(w_up, DU, pts), _int_sum(w_dn, DU, pts)
-                        tc.append(f"L{k}_coef[{cf['up']}] * (ius{dirn} * ((L{k}_beta[{dirn}] > 0.0) ? {su} : {sd}))")
+                        # the sign taken out of each sum goes into its scale (a free operand
+                        # modifier of the multiply)
+                        (gu, su), (gd, sd) = _int_sum(w_up, DU, pts), _int_sum(w_dn, DU, pts)
+                        fu, fd_ = ("n" if gu < 0 else "") + f"ius{dirn}", ("n" if gd < 0 else "") + f"ius{dirn}"
+                        tc.append(f"L{k}_coef[{cf['up']}] * ((L{k}_beta[{dirn}] > 0.0) ? ({fu} * {su}) : ({fd_} * {sd}))")
                         if k == 0:
                             fl[0] += 2 * (2 * len(w_up) - 1) + 2
                     if "ko" in cf:
