@@ -47,8 +47,8 @@ PEAK_HBM_GBS = 8000.0
 # x1.000); summary and commands in profiles/bssn_sinhspherical_192x96x96_hbm_traffic.md.
 MEASURED_TRAFFIC = {
     ("SinhSpherical_symnone_KO1_fd6", (192, 96, 96), "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_xcd"): {
-        "fetch_bytes": 2 * 1371790 * 1024.0,
-        "write_bytes": 455020 * 1024.0,
+        "fetch_bytes": 2 * 1200440 * 1024.0,
+        "write_bytes": 452411 * 1024.0,
     },
     ("SinhSpherical_symnone_KO1_fd6", (192, 96, 96), "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_xcd"): {
         "fetch_bytes": 2 * 1312480 * 1024.0,
