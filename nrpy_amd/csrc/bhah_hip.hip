@@ -279,6 +279,85 @@ bc_radiation_lines_kernel(const int *__restrict__ lines, int nlines, int ngfs, i
   }
 }
 
+// The same walk with each line's last RAD_SEG values of the gridfunction staged in LDS by
+// coalesced loads (lanes of a wave read consecutive values of four lines), instead of every
+// lane reading its own line with a separate cache line per load; the layers' results are
+// written back the same way.  Needs Nx0 >= RAD_SEG and every stencil point of the walk
+// within the segment (NG + R + 2 <= RAD_SEG, checked by the host).  Same arithmetic as
+// bc_radiation_lines_kernel.
+constexpr int RAD_SEG = 16, RAD_PITCH = RAD_SEG + 1;
+constexpr long RAD_STAGED_MIN_ITEMS = 8192;
+
+__device__ __forceinline__ double partial_r_seg(const RadStencil &st, int R, const double *seg, int Nx0, int i0,
+                                                double dx0dr, double invdxx0) {
+#pragma clang fp contract(off)
+  // one-sided towards the interior at the outer face (FACE0 = -1), as partial_r_f
+  const int off = (i0 > Nx0 - R - 1) ? (Nx0 - R - 1) - i0 : -1;
+  const int k = off + R;
+  const int c = i0 - (Nx0 - RAD_SEG);
+  double acc = 0.0;
+  for (int j = 0; j < st.n[k]; j++) acc += st.w[k][j] * seg[c + st.off[k][j]];
+  return dx0dr * (acc * invdxx0);
+}
+
+__global__ void __launch_bounds__(256)
+bc_radiation_lines_lds_kernel(const int *__restrict__ lines, int nlines, int ngfs, int NG,
+                              const double *__restrict__ gfs, double *__restrict__ rhs, const double *__restrict__ xx0,
+                              const double *__restrict__ wavespeed, const double *__restrict__ finf, RadStencil st,
+                              int R, int coord, double AMPL, double SINHW, int Nx0, long s1, long s2, long Ntot,
+                              double invdxx0) {
+#pragma clang fp contract(off)
+  __shared__ double seg[256 * RAD_PITCH];
+  __shared__ long row[256];  // gf * Ntot + line base of each item of the block
+  const int tid = (int)threadIdx.x;
+  const long nitems = (long)nlines * ngfs, t0 = (long)blockIdx.x * 256, t = t0 + tid;
+  const bool active = t < nitems;
+  if (active) {
+    const int l = (int)(t % nlines), gf = (int)(t / nlines);
+    row[tid] = gf * Ntot + s1 * lines[2 * l] + s2 * lines[2 * l + 1];
+  }
+  __syncthreads();
+  for (int j = 0; j < RAD_SEG; j++) {
+    const int e = tid + 256 * j, it = e / RAD_SEG, x = e % RAD_SEG;
+    if (t0 + it < nitems) seg[it * RAD_PITCH + x] = gfs[row[it] + (Nx0 - RAD_SEG) + x];
+  }
+  __syncthreads();
+  if (active) {
+    const double *f = &seg[tid * RAD_PITCH];
+    const long base = row[tid];
+    const int gf = (int)(t / nlines);
+    const double c = wavespeed[gf], f_inf = finf[gf];
+    double dtf_int = rhs[base + Nx0 - NG - 1];
+    double r_int, dxr_int;
+    r_and_dx0dr(coord, AMPL, SINHW, xx0[Nx0 - NG - 1], r_int, dxr_int);
+    double v[8];
+    for (int gz = 0; gz < NG; gz++) {
+      const int i0 = Nx0 - NG + gz, j0 = i0 - 1;
+      double r, dxr;
+      r_and_dx0dr(coord, AMPL, SINHW, xx0[i0], r, dxr);
+      const double dfr = partial_r_seg(st, R, f, Nx0, i0, dxr, invdxx0);
+      const double dfr_int = partial_r_seg(st, R, f, Nx0, j0, dxr_int, invdxx0);
+      const double dtf_int_out = -c * (dfr_int + (f[j0 - (Nx0 - RAD_SEG)] - f_inf) / r_int);
+      const double k = r_int * r_int * r_int * (dtf_int - dtf_int_out);
+      const double rinv = 1.0 / r;
+      const double dtf_out = -c * (dfr + (f[i0 - (Nx0 - RAD_SEG)] - f_inf) * rinv);
+      v[gz] = dtf_out + k * rinv * rinv * rinv;
+      dtf_int = v[gz];
+      r_int = r;
+      dxr_int = dxr;
+    }
+    // this thread alone read its segment: its results take the segment's first NG slots
+    for (int gz = 0; gz < NG; gz++) seg[tid * RAD_PITCH + gz] = v[gz];
+  }
+  __syncthreads();
+  for (int j = 0; j < NG; j++) {
+    const int e = tid + 256 * j, it = e / NG, gz = e % NG;
+    if (t0 + it < nitems) rhs[row[it] + (Nx0 - NG) + gz] = seg[it * RAD_PITCH + gz];
+  }
+}
+
+constexpr int BC_INNER_BATCH = 12;  // gridfunctions in flight per thread in the inner-parity copy
+
 // Batched forms over an ensemble: one record per (member, boundary point), one
 // thread per (record, gridfunction); member arrays are picked by index.
 struct ens_outer_pt {
@@ -313,9 +392,14 @@ ens_bc_inner_kernel(const bhah_member *__restrict__ mem, const ens_inner_pt *__r
   const bhah_member *M = &mem[e.m];
   double *f = M->arr[sel];
   const long Ntot = M->kp.Ntot;
-  for (int gf = 0; gf < ngfs; gf++) {
-    const double v = f[gf * Ntot + e.src];
-    f[gf * Ntot + e.dst] = ((e.mask >> gf_parity[gf]) & 1u) ? -v : v;
+  for (int g0 = 0; g0 < ngfs; g0 += BC_INNER_BATCH) {  // loads of a batch before its stores
+    double v[BC_INNER_BATCH];
+#pragma unroll
+    for (int j = 0; j < BC_INNER_BATCH; j++)
+      if (g0 + j < ngfs) v[j] = f[(g0 + j) * Ntot + e.src];
+#pragma unroll
+    for (int j = 0; j < BC_INNER_BATCH; j++)
+      if (g0 + j < ngfs) f[(g0 + j) * Ntot + e.dst] = ((e.mask >> gf_parity[g0 + j]) & 1u) ? -v[j] : v[j];
   }
 }
 
@@ -344,9 +428,17 @@ bc_inner_kernel(const int *__restrict__ dst, const int *__restrict__ src, const 
   if (p >= npts) return;
   const long d = dst[p], sr = src[p];
   const unsigned m = mask[p];
-  for (int gf = 0; gf < ngfs; gf++) {
-    const double v = gfs[gf * Ntot + sr];
-    gfs[gf * Ntot + d] = ((m >> gf_parity[gf]) & 1u) ? -v : v;
+  // Sources are interior points, never another boundary point's destination, so the loads
+  // of a batch of gridfunctions may all be issued before its stores (the compiler cannot
+  // know that the stores do not alias later loads of the same array).
+  for (int g0 = 0; g0 < ngfs; g0 += BC_INNER_BATCH) {
+    double v[BC_INNER_BATCH];
+#pragma unroll
+    for (int j = 0; j < BC_INNER_BATCH; j++)
+      if (g0 + j < ngfs) v[j] = gfs[(g0 + j) * Ntot + sr];
+#pragma unroll
+    for (int j = 0; j < BC_INNER_BATCH; j++)
+      if (g0 + j < ngfs) gfs[(g0 + j) * Ntot + d] = ((m >> gf_parity[g0 + j]) & 1u) ? -v[j] : v[j];
   }
 }
 
@@ -1037,9 +1129,20 @@ int bhah_hip_apply_bcs_outerradiation_and_inner(bhah_patch *P, const double *gfs
   if (P->n_rad_lines) {
     dim3 g, b;
     launch_1d((long)P->n_rad_lines * P->ngfs, g, b);
-    hipLaunchKernelGGL(bc_radiation_lines_kernel, g, b, 0, P->stream, P->d_rad_lines, P->n_rad_lines, P->ngfs, P->NG,
-                       gfs, rhs, P->d_xx0, P->d_wavespeed, P->d_finf, st, R, P->d.coord, P->d.grid_physical_size,
-                       P->d.SINHW, P->Nx[0], (long)P->Nx[0], (long)P->Nx[0] * P->Nx[1], P->Ntot, P->invdxx[0]);
+    // Staged form from a few thousand (line, gridfunction) items up (259,584 items at
+    // 192x96x96: 71 -> 26 us per launch). Small patches keep the one-pass form, whose launch
+    // has no block-wide steps: at 3,456 items (72x12x2) the staged one took ~3 us longer.
+    const long items = (long)P->n_rad_lines * P->ngfs;
+    if (items >= RAD_STAGED_MIN_ITEMS && P->Nx[0] >= RAD_SEG && P->NG + R + 2 <= RAD_SEG && P->NG <= 8)
+      hipLaunchKernelGGL(bc_radiation_lines_lds_kernel, dim3((unsigned)(((long)P->n_rad_lines * P->ngfs + 255) / 256)),
+                         dim3(256), 0, P->stream, P->d_rad_lines, P->n_rad_lines, P->ngfs, P->NG, gfs, rhs, P->d_xx0,
+                         P->d_wavespeed, P->d_finf, st, R, P->d.coord, P->d.grid_physical_size, P->d.SINHW, P->Nx[0],
+                         (long)P->Nx[0], (long)P->Nx[0] * P->Nx[1], P->Ntot, P->invdxx[0]);
+    else
+      hipLaunchKernelGGL(bc_radiation_lines_kernel, g, b, 0, P->stream, P->d_rad_lines, P->n_rad_lines, P->ngfs,
+                         P->NG, gfs, rhs, P->d_xx0, P->d_wavespeed, P->d_finf, st, R, P->d.coord,
+                         P->d.grid_physical_size, P->d.SINHW, P->Nx[0], (long)P->Nx[0], (long)P->Nx[0] * P->Nx[1],
+                         P->Ntot, P->invdxx[0]);
     HIP_TRY(hipGetLastError());
     return bhah_hip_apply_bcs_inner_only(P, rhs);
   }
