@@ -927,16 +927,18 @@ FULL_CANDIDATES = {
     "rhs": ["p1_lb1", "p2_lb1_sj_lf_pf16_u32_ck6_w2", "p1_lb1_sj_L16x4x4x1_d2_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_xcd",
             "p1_lb1_sj_L16x4x4x1_d2_dma_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_xcd", "p1_lb1_sj_L16x4x4x1_d4_dma_rb1_xcd",
             "p1_lb1_sj_L16x4x4x1_d2_dma_rb1_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_xcd",
-            "p1_lb1_sj_L16x4x4x1_d4_dma_rb1_io_xcd"],
+            "p1_lb1_sj_L16x4x4x1_d4_dma_rb1_io_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg4_xcd",
+            "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg8_xcd"],
     "ricci": ["r1_lb1", "r2_lb1_sj_lf_pf12_zs16", "r1_lb1_sj_L16x4x4x1_g3_d2_xcd", "r1_lb1_sj_L16x4x4x1_d3_dma_xcd",
               "r1_lb1_sj_L16x4x4x1_d2_dma_xcd", "r1_lb1_sj_L16x4x4x1_d3_dma_rb1_xcd", "r1_lb1_sj_L16x4x4x1_d4_dma_rb1_xcd",
               "r1_lb1_sj_L16x4x4x1_g3_d2_dma_rb1_xcd", "r1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_xcd",
-              "r1_lb1_sj_L16x4x4x1_d2_dma_io_xcd"],
+              "r1_lb1_sj_L16x4x4x1_d2_dma_io_xcd", "r1_lb1_sj_L16x4x4x1_d2_dma_io_pg2_xcd"],
 }
 SMALL_CANDIDATES = {"rhs": ["p1_lb1", "p1_lb1_sl", "p3_lb1_sl", "p1_lb1_sl_ln_pf16", "p3_lb1_sl_ln_pf16",
                             "p3_lb1_sl_ln_pf16_u32", "p3_lb1_sl_ln_pf16_u32_ck8", "p3_lb1_sl_lf_pf16_u32_ck8",
                             "p1_lb1_sl_lf_pf16_u32", "p1_lb1_sl_lf_pf16_u32_w2", "p1_lb1_sj_L32x8x1x1_d2_dma",
-                            "p1_lb1_sj_L16x8x2x1_d2_dma", "p1_lb1_sj_L16x8x2x1_d3_dma_rb1"],
+                            "p1_lb1_sj_L16x8x2x1_d2_dma", "p1_lb1_sj_L16x8x2x1_d3_dma_rb1",
+                            "p1_lb1_sj_L16x8x2x1_d3_dma_rb1_pg8"],
                     "ricci": ["r1_lb1", "r1_lb1_sl", "r2_lb1_sl", "r1_lb1_pf16", "r2_lb1_sl_pf16", "r1_lb1_pf16_u32",
                               "r2_lb1_sl_pf16_u32", "r2_lb1_sl_pf16_u32_ck8", "r2_lb1_sl_lf_pf16_u32",
                               "r2_lb1_sl_lf_pf16_u32_w2", "r2_lb1_sj_L16x4x4x1", "r1_lb1_sj_L16x4x4x1_d2_dma_xcd",
@@ -954,6 +956,8 @@ def _subspec(spec: specs.Spec, name: str, gfs: List[str]) -> specs.Spec:
 # An empty register-constrained asm: the value must exist in a VGPR at this point.
 LDS_PIN_MACRO = """#ifndef BHAH_PIN
 #define BHAH_PIN(x) asm volatile("" : "+v"(x))
+// several values at once: BHAH_PINS("+v"(a), "+v"(b), ...)
+#define BHAH_PINS(...) asm volatile("" : __VA_ARGS__)
 #endif
 """
 
@@ -1079,7 +1083,8 @@ def _lds_geometry(tok: str) -> Tuple[int, int, int, int]:
 def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str], geom: Tuple[int, int, int, int],
                min_waves: int, scope: str, lv: List[str], xcd: bool = False,
                depth: int = 1, zfirst: bool = False, group: int = 1, dma: bool = False,
-               one_barrier: bool = False, ensemble: str = "", interleave: bool = False) -> Tuple[str, str, int]:
+               one_barrier: bool = False, ensemble: str = "", interleave: bool = False,
+               pin_group: int = -1) -> Tuple[str, str, int]:
     """One workgroup-tiled kernel.  Each gridfunction with derivatives is staged, one at a
     time, from global memory into a halo box in LDS: ``group`` gridfunctions per
     barrier-separated phase, with the loads of the next ``depth`` phases in flight while
@@ -1226,18 +1231,33 @@ def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str],
     def emit_phase(ph, gfs):
         """Derivatives and linear-term contributions of the phase's gridfunctions."""
         for q, g in enumerate(gfs):
+            pins: List[str] = []
+
+            def pin(x):
+                # each value is pinned to a register once formed: otherwise the compiler
+                # sinks the stencil sums into the guarded algebra below and keeps every raw
+                # LDS value alive across the staging barriers instead.  A pin is a volatile
+                # asm, which no memory access crosses; with pin_group >= 0 one pin covers
+                # pin_group formed values (0: all of the staged gridfunction's), so that the
+                # LDS reads of several stencils can be in flight together.
+                if pin_group < 0:
+                    a(f"  BHAH_PIN({x});")
+                    return
+                if x not in pins:
+                    pins.append(x)
+                if pin_group and len(pins) >= pin_group:
+                    a(f"  BHAH_PINS({', '.join(f'{chr(34)}+v{chr(34)}({y})' for y in pins)});")
+                    pins.clear()
+
             for k in range(P):
                 st = _LdsEmitter(sub, fd_order, f"c{k}", BX, BY, box(ph, q))
-                # each value is pinned to a register where it is formed: otherwise the
-                # compiler sinks the stencil sums into the guarded algebra below and keeps
-                # every raw LDS value alive across the staging barriers instead
                 if g in ovr and g in used:
                     a(f"  double V{k}_{g} = {st.load(g, (0, 0, 0))};")
-                    a(f"  BHAH_PIN(V{k}_{g});")
+                    pin(f"V{k}_{g}")
                 for d in dused:
                     if d.gf == g:
                         a(f"  double V{k}_{d.symbol} = {st.expr(d)};")
-                        a(f"  BHAH_PIN(V{k}_{d.symbol});")
+                        pin(f"V{k}_{d.symbol}")
                 for dirn in range(3):
                     key = (g, dirn)
                     if key not in line_out:
@@ -1259,7 +1279,9 @@ def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str],
                         if k == 0:
                             fl[0] += 3 * (len(w_ko) // 2) + 3
                     a(f"  {acc} += {' + '.join(tc)};")
-                    a(f"  BHAH_PIN({acc});")
+                    pin(acc)
+            for i in range(0, len(pins), 24):
+                a(f"  BHAH_PINS({', '.join(f'{chr(34)}+v{chr(34)}({x})' for x in pins[i:i + 24])});")
 
     if dma:
         # every lane of every wave issues NJ loads per box (padded boxes), so the number of
@@ -1430,7 +1452,8 @@ def candidate_tu(v: specs.Variant, fd_order: int, kind: str, cand: str, ensemble
                                          zfirst="zf" in opts,
                                          group=next((int(o[1:]) for o in opts if o[0] == "g" and o[1:].isdigit()), 1),
                                          dma="dma" in opts, one_barrier="rb1" in opts, ensemble=ename,
-                                         interleave="io" in opts)
+                                         interleave="io" in opts,
+                                         pin_group=next((int(o[2:] or 0) for o in opts if o[:2] == "pg" and o[2:].isdigit() or o == "pg"), -1))
             kernels.append(src)
             calls.append(launch)
             total += fl

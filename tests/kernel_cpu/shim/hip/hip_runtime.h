@@ -17,6 +17,7 @@ inline dim3 blockIdx, blockDim, gridDim;
 inline thread_local dim3 threadIdx;
 inline int min(int a, int b) { return a < b ? a : b; }
 #define BHAH_PIN(x) ((void)0)
+#define BHAH_PINS(...) ((void)0)
 #define __builtin_nontemporal_load(p) (*(p))
 #define __builtin_nontemporal_store(v, p) (*(p) = (v))
 #define __shared__ static
