@@ -927,8 +927,9 @@ FULL_CANDIDATES = {
     "rhs": ["p1_lb1", "p2_lb1_sj_lf_pf16_u32_ck6_w2", "p1_lb1_sj_L16x4x4x1_d2_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_xcd",
             "p1_lb1_sj_L16x4x4x1_d2_dma_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_xcd", "p1_lb1_sj_L16x4x4x1_d4_dma_rb1_xcd",
             "p1_lb1_sj_L16x4x4x1_d2_dma_rb1_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_xcd",
-            "p1_lb1_sj_L16x4x4x1_d4_dma_rb1_io_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg4_xcd",
-            "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg8_xcd"],
+            "p1_lb1_sj_L16x4x4x1_d4_dma_rb1_io_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg8_xcd",
+            "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg8_ub_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg12_ub_xcd",
+            "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg_ub_xcd"],
     "ricci": ["r1_lb1", "r2_lb1_sj_lf_pf12_zs16", "r1_lb1_sj_L16x4x4x1_g3_d2_xcd", "r1_lb1_sj_L16x4x4x1_d3_dma_xcd",
               "r1_lb1_sj_L16x4x4x1_d2_dma_xcd", "r1_lb1_sj_L16x4x4x1_d3_dma_rb1_xcd", "r1_lb1_sj_L16x4x4x1_d4_dma_rb1_xcd",
               "r1_lb1_sj_L16x4x4x1_g3_d2_dma_rb1_xcd", "r1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_xcd",
@@ -938,7 +939,7 @@ SMALL_CANDIDATES = {"rhs": ["p1_lb1", "p1_lb1_sl", "p3_lb1_sl", "p1_lb1_sl_ln_pf
                             "p3_lb1_sl_ln_pf16_u32", "p3_lb1_sl_ln_pf16_u32_ck8", "p3_lb1_sl_lf_pf16_u32_ck8",
                             "p1_lb1_sl_lf_pf16_u32", "p1_lb1_sl_lf_pf16_u32_w2", "p1_lb1_sj_L32x8x1x1_d2_dma",
                             "p1_lb1_sj_L16x8x2x1_d2_dma", "p1_lb1_sj_L16x8x2x1_d3_dma_rb1",
-                            "p1_lb1_sj_L16x8x2x1_d3_dma_rb1_pg8"],
+                            "p1_lb1_sj_L16x8x2x1_d3_dma_rb1_pg8", "p1_lb1_sj_L16x8x2x1_d3_dma_rb1_pg8_ub"],
                     "ricci": ["r1_lb1", "r1_lb1_sl", "r2_lb1_sl", "r1_lb1_pf16", "r2_lb1_sl_pf16", "r1_lb1_pf16_u32",
                               "r2_lb1_sl_pf16_u32", "r2_lb1_sl_pf16_u32_ck8", "r2_lb1_sl_lf_pf16_u32",
                               "r2_lb1_sl_lf_pf16_u32_w2", "r2_lb1_sj_L16x4x4x1", "r1_lb1_sj_L16x4x4x1_d2_dma_xcd",
@@ -1084,7 +1085,7 @@ def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str],
                min_waves: int, scope: str, lv: List[str], xcd: bool = False,
                depth: int = 1, zfirst: bool = False, group: int = 1, dma: bool = False,
                one_barrier: bool = False, ensemble: str = "", interleave: bool = False,
-               pin_group: int = -1) -> Tuple[str, str, int]:
+               pin_group: int = -1, branchless_upwind: bool = False) -> Tuple[str, str, int]:
     """One workgroup-tiled kernel.  Each gridfunction with derivatives is staged, one at a
     time, from global memory into a halo box in LDS: ``group`` gridfunctions per
     barrier-separated phase, with the loads of the next ``depth`` phases in flight while
@@ -1271,7 +1272,15 @@ def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str],
                         # modifier of the multiply)
                         (gu, su), (gd, sd) = _int_sum(w_up, DU, pts), _int_sum(w_dn, DU, pts)
                         fu, fd_ = ("n" if gu < 0 else "") + f"ius{dirn}", ("n" if gd < 0 else "") + f"ius{dirn}"
-                        tc.append(f"L{k}_coef[{cf['up']}] * ((L{k}_beta[{dirn}] > 0.0) ? ({fu} * {su}) : ({fd_} * {sd}))")
+                        if branchless_upwind:
+                            # both one-sided sums formed (from the points the KO sum reads
+                            # anyway) and one selected: no branch splits the scheduling region
+                            a(f"  const double U{k}_{g}_{dirn} = {fu} * {su}, D{k}_{g}_{dirn} = {fd_} * {sd};")
+                            tc.append(f"L{k}_coef[{cf['up']}] * ((L{k}_beta[{dirn}] > 0.0) ? U{k}_{g}_{dirn} : "
+                                      f"D{k}_{g}_{dirn})")
+                        else:
+                            tc.append(f"L{k}_coef[{cf['up']}] * ((L{k}_beta[{dirn}] > 0.0) ? ({fu} * {su}) : "
+                                      f"({fd_} * {sd}))")
                         if k == 0:
                             fl[0] += 2 * (2 * len(w_up) - 1) + 2
                     if "ko" in cf:
@@ -1453,6 +1462,7 @@ def candidate_tu(v: specs.Variant, fd_order: int, kind: str, cand: str, ensemble
                                          group=next((int(o[1:]) for o in opts if o[0] == "g" and o[1:].isdigit()), 1),
                                          dma="dma" in opts, one_barrier="rb1" in opts, ensemble=ename,
                                          interleave="io" in opts,
+                                         branchless_upwind="ub" in opts,
                                          pin_group=next((int(o[2:] or 0) for o in opts if o[:2] == "pg" and o[2:].isdigit() or o == "pg"), -1))
             kernels.append(src)
             calls.append(launch)

@@ -122,7 +122,8 @@ ENS_DRIVER = os.path.join(HERE, "kernel_cpu", "ens_driver.cpp")
                                             ("rhs", "p3_lb1_sl_ln_pf16", (64, 4, 1)),
                                             ("ricci", "r1_lb1_sj_L16x8x2x1_d2_dma", (16, 8, 2)),
                                             ("rhs", "p1_lb1_sj_L16x8x2x1_d3_dma_rb1", (16, 8, 2)),
-                                            ("rhs", "p1_lb1_sj_L16x8x2x1_d3_dma_rb1_pg8", (16, 8, 2))])
+                                            ("rhs", "p1_lb1_sj_L16x8x2x1_d3_dma_rb1_pg8", (16, 8, 2)),
+                                            ("rhs", "p1_lb1_sj_L16x8x2x1_d3_dma_rb1_pg8_ub", (16, 8, 2))])
 def test_ensemble_kernels_on_cpu(oracle_lib, kind, cand, tile):
     """Batched-ensemble form of a schedule over members of different grids, with a
     persistent grid smaller than the tile list, matches the oracle per member (LDS-tiled
