@@ -1566,6 +1566,61 @@ bssn_rk_det_{tag}(const double *__restrict__ kin, const double *y_n, double *__r
   for (int g = 0; g < NGF; g++) __builtin_nontemporal_store(nxt[g], &out[g * Ntot + idx]);
 }}
 """
+    # Two consecutive points per thread with 16-byte loads and stores: twice the bytes per
+    # request of every array stream.  Same arithmetic per point as the pass above; the
+    # launcher takes it when Ntot is even and every array is 16-byte aligned.
+    rkdet += f"""
+typedef double bhah_d2 __attribute__((ext_vector_type(2)));
+extern "C" __global__ void __launch_bounds__(256)
+bssn_rk_det2_{tag}(const double *__restrict__ kin, const double *y_n, double *__restrict__ y_run, double *out,
+                   const double *__restrict__ rfm0, const double *__restrict__ rfm1, const bhah_kparams p) {{
+  const long Ntot = p.Ntot;
+  const long idx2 = ((long)blockIdx.x * 256 + threadIdx.x) * 2;
+  if (idx2 >= Ntot) return;
+  constexpr int NGF = {ngf};
+  const bhah_d2 *K2 = (const bhah_d2 *)(kin + idx2), *YN2 = (const bhah_d2 *)(y_n + idx2);
+  bhah_d2 *YR2 = (bhah_d2 *)(y_run + idx2), *OUT2 = (bhah_d2 *)(out + idx2);
+  const long S2 = Ntot / 2;  // gridfunction stride in pairs
+  // RK4 substep update (register_all_MoL_step_forward_in_time.c:4-103), in registers.
+  const double dt = p.dt;
+  bhah_d2 nx2[NGF];
+  if (p.substep == 1) {{
+#pragma unroll
+    for (int g = 0; g < NGF; g++) {{
+      const bhah_d2 k = __builtin_nontemporal_load(&K2[g * S2]);
+      __builtin_nontemporal_store((1.0 / 6.0) * dt * k, &YR2[g * S2]);
+      nx2[g] = (1.0 / 2.0) * dt * k + __builtin_nontemporal_load(&YN2[g * S2]);
+    }}
+  }} else if (p.substep == 4) {{
+#pragma unroll
+    for (int g = 0; g < NGF; g++) {{
+      const bhah_d2 k = __builtin_nontemporal_load(&K2[g * S2]);
+      nx2[g] = (1.0 / 6.0) * dt * k + __builtin_nontemporal_load(&YN2[g * S2]) + __builtin_nontemporal_load(&YR2[g * S2]);
+    }}
+  }} else {{
+    const double a = (p.substep == 2) ? (1.0 / 2.0) : 1.0;
+#pragma unroll
+    for (int g = 0; g < NGF; g++) {{
+      const bhah_d2 k = __builtin_nontemporal_load(&K2[g * S2]);
+      __builtin_nontemporal_store((1.0 / 3.0) * dt * k + __builtin_nontemporal_load(&YR2[g * S2]), &YR2[g * S2]);
+      nx2[g] = a * dt * k + __builtin_nontemporal_load(&YN2[g * S2]);
+    }}
+  }}
+#pragma unroll
+  for (int h = 0; h < 2; h++) {{
+    const long idx = idx2 + h;
+    const int i0 = (int)(idx % p.Nx0), i1 = (int)((idx / p.Nx0) % p.Nx1);
+    double nxt[NGF];
+#pragma unroll
+    for (int g = 0; g < NGF; g++) nxt[g] = nx2[g][h];
+{det}
+#pragma unroll
+    for (int g = 0; g < NGF; g++) nx2[g][h] = nxt[g];
+  }}
+#pragma unroll
+  for (int g = 0; g < NGF; g++) __builtin_nontemporal_store(nx2[g], &OUT2[g * S2]);
+}}
+"""
     ens_rk = ""
     if ensemble:
         ens_rk = f"""
@@ -1638,8 +1693,14 @@ extern "C" hipError_t launch_bssn_rk_det_{tag}(dim3 g, dim3 b, hipStream_t s, co
     double *y_run, double *out, const double *rfm0, const double *rfm1, bhah_kparams p) {{
   (void)g;
   (void)b;
-  hipLaunchKernelGGL(bssn_rk_det_{tag}, dim3((unsigned)((p.Ntot + 255) / 256)), dim3(256), 0, s, kin, y_n, y_run, out,
-                     rfm0, rfm1, p);
+  const bool pairs = !(p.Ntot & 1) && !((reinterpret_cast<uintptr_t>(kin) | reinterpret_cast<uintptr_t>(y_n) |
+                                         reinterpret_cast<uintptr_t>(y_run) | reinterpret_cast<uintptr_t>(out)) & 15);
+  if (pairs)
+    hipLaunchKernelGGL(bssn_rk_det2_{tag}, dim3((unsigned)((p.Ntot / 2 + 255) / 256)), dim3(256), 0, s, kin, y_n, y_run,
+                       out, rfm0, rfm1, p);
+  else
+    hipLaunchKernelGGL(bssn_rk_det_{tag}, dim3((unsigned)((p.Ntot + 255) / 256)), dim3(256), 0, s, kin, y_n, y_run,
+                       out, rfm0, rfm1, p);
   return hipGetLastError();
 }}
 extern "C" hipError_t launch_bssn_constraints_{tag}(dim3 g, dim3 b, hipStream_t s, const double *in,
