@@ -45,16 +45,24 @@ def sources() -> List[str]:
     return gen + [os.path.join(CSRC, "bhah_hip.hip")]
 
 
+def _flags(src: str) -> List[str]:
+    """Per-source flags: schedules named with ``_mi`` are scheduled by LLVM's max-ILP
+    strategy, which keeps fewer values in flight through the point algebra."""
+    if re.search(r"_mi(_|\.hip$)", os.path.basename(src)):
+        return [*HIPFLAGS, "-mllvm", "-amdgpu-sched-strategy=max-ilp"]
+    return HIPFLAGS
+
+
 def _compile(src: str) -> str:
     obj = os.path.join(OBJDIR, os.path.basename(src).replace(".hip", ".o"))
-    cmd = [HIPCC, *HIPFLAGS, "-c", src, "-o", obj]
+    cmd = [HIPCC, *_flags(src), "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr[-4000:]}")
     if re.search(r"_pf|_L\d", os.path.basename(src)):
         # early-issued stencil loads: verify the vmcnt range on the emitted assembly
         asm = obj[:-2] + ".s"
-        r = subprocess.run([HIPCC, *HIPFLAGS, "--cuda-device-only", "-S", src, "-o", asm], capture_output=True,
+        r = subprocess.run([HIPCC, *_flags(src), "--cuda-device-only", "-S", src, "-o", asm], capture_output=True,
                            text=True)
         if r.returncode != 0:
             raise RuntimeError(f"hipcc -S failed for {src}:\n{r.stderr[-4000:]}")

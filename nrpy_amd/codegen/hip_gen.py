@@ -929,21 +929,25 @@ FULL_CANDIDATES = {
             "p1_lb1_sj_L16x4x4x1_d2_dma_rb1_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_xcd",
             "p1_lb1_sj_L16x4x4x1_d4_dma_rb1_io_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg8_xcd",
             "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg8_ub_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg12_ub_xcd",
-            "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg_ub_xcd"],
+            "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg_ub_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg12_ub_mi_xcd",
+            "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg8_ub_mi_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_ub_mi_xcd"],
     "ricci": ["r1_lb1", "r2_lb1_sj_lf_pf12_zs16", "r1_lb1_sj_L16x4x4x1_g3_d2_xcd", "r1_lb1_sj_L16x4x4x1_d3_dma_xcd",
               "r1_lb1_sj_L16x4x4x1_d2_dma_xcd", "r1_lb1_sj_L16x4x4x1_d3_dma_rb1_xcd", "r1_lb1_sj_L16x4x4x1_d4_dma_rb1_xcd",
               "r1_lb1_sj_L16x4x4x1_g3_d2_dma_rb1_xcd", "r1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_xcd",
-              "r1_lb1_sj_L16x4x4x1_d2_dma_io_xcd", "r1_lb1_sj_L16x4x4x1_d2_dma_io_pg2_xcd"],
+              "r1_lb1_sj_L16x4x4x1_d2_dma_io_xcd", "r1_lb1_sj_L16x4x4x1_d2_dma_io_pg2_xcd",
+              "r1_lb1_sj_L16x4x4x1_d2_dma_io_mi_xcd", "r1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_mi_xcd"],
 }
 SMALL_CANDIDATES = {"rhs": ["p1_lb1", "p1_lb1_sl", "p3_lb1_sl", "p1_lb1_sl_ln_pf16", "p3_lb1_sl_ln_pf16",
                             "p3_lb1_sl_ln_pf16_u32", "p3_lb1_sl_ln_pf16_u32_ck8", "p3_lb1_sl_lf_pf16_u32_ck8",
                             "p1_lb1_sl_lf_pf16_u32", "p1_lb1_sl_lf_pf16_u32_w2", "p1_lb1_sj_L32x8x1x1_d2_dma",
                             "p1_lb1_sj_L16x8x2x1_d2_dma", "p1_lb1_sj_L16x8x2x1_d3_dma_rb1",
-                            "p1_lb1_sj_L16x8x2x1_d3_dma_rb1_pg8", "p1_lb1_sj_L16x8x2x1_d3_dma_rb1_pg8_ub"],
+                            "p1_lb1_sj_L16x8x2x1_d3_dma_rb1_pg8", "p1_lb1_sj_L16x8x2x1_d3_dma_rb1_pg8_ub",
+                            "p1_lb1_sj_L16x8x2x1_d3_dma_rb1_pg8_ub_mi"],
                     "ricci": ["r1_lb1", "r1_lb1_sl", "r2_lb1_sl", "r1_lb1_pf16", "r2_lb1_sl_pf16", "r1_lb1_pf16_u32",
                               "r2_lb1_sl_pf16_u32", "r2_lb1_sl_pf16_u32_ck8", "r2_lb1_sl_lf_pf16_u32",
                               "r2_lb1_sl_lf_pf16_u32_w2", "r2_lb1_sj_L16x4x4x1", "r1_lb1_sj_L16x4x4x1_d2_dma_xcd",
-                              "r1_lb1_sj_L32x8x1x1_d2_dma", "r1_lb1_sj_L16x8x2x1_d2_dma", "r1_lb1_sj_L16x8x2x1_d3_dma_rb1"]}
+                              "r1_lb1_sj_L32x8x1x1_d2_dma", "r1_lb1_sj_L16x8x2x1_d2_dma", "r1_lb1_sj_L16x8x2x1_d3_dma_rb1",
+                              "r1_lb1_sj_L16x8x2x1_d3_dma_rb1_mi"]}
 
 CAND_SIG = ("dim3 g, dim3 b, hipStream_t s, const double *in, const double *aux, double *out, "
             "const double *rfm0, const double *rfm1, bhah_kparams p")
