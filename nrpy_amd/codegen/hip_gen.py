@@ -929,13 +929,13 @@ FULL_CANDIDATES = {
             "p1_lb1_sj_L16x4x4x1_d2_dma_rb1_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_xcd",
             "p1_lb1_sj_L16x4x4x1_d4_dma_rb1_io_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg8_xcd",
             "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg8_ub_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg12_ub_xcd",
-            "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg_ub_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg12_ub_mi_xcd",
-            "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg8_ub_mi_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_ub_mi_xcd"],
+            "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg12_ub_mi_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg_ub_mi_xcd"],
     "ricci": ["r1_lb1", "r2_lb1_sj_lf_pf12_zs16", "r1_lb1_sj_L16x4x4x1_g3_d2_xcd", "r1_lb1_sj_L16x4x4x1_d3_dma_xcd",
               "r1_lb1_sj_L16x4x4x1_d2_dma_xcd", "r1_lb1_sj_L16x4x4x1_d3_dma_rb1_xcd", "r1_lb1_sj_L16x4x4x1_d4_dma_rb1_xcd",
               "r1_lb1_sj_L16x4x4x1_g3_d2_dma_rb1_xcd", "r1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_xcd",
               "r1_lb1_sj_L16x4x4x1_d2_dma_io_xcd", "r1_lb1_sj_L16x4x4x1_d2_dma_io_pg2_xcd",
-              "r1_lb1_sj_L16x4x4x1_d2_dma_io_mi_xcd", "r1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_mi_xcd"],
+              "r1_lb1_sj_L16x4x4x1_d2_dma_io_mi_xcd", "r1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_mi_xcd",
+              "r1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg2_mi_xcd"],
 }
 SMALL_CANDIDATES = {"rhs": ["p1_lb1", "p1_lb1_sl", "p3_lb1_sl", "p1_lb1_sl_ln_pf16", "p3_lb1_sl_ln_pf16",
                             "p3_lb1_sl_ln_pf16_u32", "p3_lb1_sl_ln_pf16_u32_ck8", "p3_lb1_sl_lf_pf16_u32_ck8",
