@@ -1,7 +1,9 @@
 """Summarise rocprofv3 output into per-kernel tables.
 
-    python -m nrpy_amd.profsummary gpurun_out/prof/run_results.db      (kernel trace)
-    python -m nrpy_amd.profsummary gpurun_out/pmc/run_counter_collection.csv   (--pmc, csv)
+    python -m nrpy_amd.profsummary <dir>/run_results.db      (kernel trace)
+    python -m nrpy_amd.profsummary <dir>/run_counter_collection.csv   (--pmc, csv)
+
+where <dir> is the rocprofv3 output directory (``-d``) and ``run`` its output name (``-o``).
 """
 
 from __future__ import annotations
