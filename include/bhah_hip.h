@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define BHAH_HIP_ABI_VERSION 1
+#define BHAH_HIP_ABI_VERSION 2
 
 enum {
   BHAH_OK = 0,
@@ -53,6 +53,10 @@ enum { BHAH_KERNEL_RHS = 0, BHAH_KERNEL_RICCI = 1 };
 #define BHAH_NUM_EVOL_GFS_BSSN 24
 #define BHAH_NUM_AUXEVOL_GFS_BSSN 6
 #define BHAH_NUM_EVOL_GFS_WAVE 2
+/* Diagnostic gridfunctions written by the constraint kernels, in this order: H, MU0, MU1,
+ * MU2, MSQUARED, LAMBDA_CONSTRAINT. */
+#define BHAH_NUM_DIAG_GFS 6
+enum { BHAH_DIAG_H = 0, BHAH_DIAG_MU0 = 1, BHAH_DIAG_MSQUARED = 4, BHAH_DIAG_LAMBDA_CONSTRAINT = 5 };
 
 /* Plain descriptor replacing commondata_struct/params_struct for one patch. */
 typedef struct bhah_patch_desc {
@@ -70,6 +74,7 @@ typedef struct bhah_patch_desc {
   double eta;                /* Gamma-driver damping */
   double KO_strength_gauge, KO_strength_nongauge;
   double wavespeed;          /* scalar wave speed */
+  int setup_on_device;       /* 0: set-up on the host (the reference's loops); 1: on the device */
 } bhah_patch_desc;
 
 typedef struct bhah_patch bhah_patch;
@@ -84,9 +89,34 @@ typedef struct bhah_MoL_gridfunctions {
   double *auxevol_gfs;
 } bhah_MoL_gridfunctions;
 
+/* The time members of commondata_struct that MoL_step_forward_in_time updates
+ * (nrpy/infrastructures/BHaH/MoLtimestepping/MoL_step_forward_in_time.py:473-481;
+ * t_0 and nn_0 are reset after reading a checkpoint, read_checkpoint__openmp.c:183-185). */
+typedef struct bhah_clock {
+  double time, t_0, dt;
+  int nn, nn_0;
+} bhah_clock;
+
+/* Volume integrals of the reference's two diagnostics recipes
+ * (nrpy/infrastructures/BHaH/general_relativity/diagnostics_volume_integration.py:177-207):
+ * recipe 0 "whole_domain", recipe 1 "sphere_R_gt_8" (points with |x| > 8 about the origin).
+ * Integrand k is f_k^2 for f = H, MSQUARED, LAMBDA_CONSTRAINT (is_squared = 1), with the
+ * proper volume element dV = sqrt(det gammahat) |dxx0 dxx1 dxx2| at interior points
+ * (diagnostics_volume_integration_helpers.h:396-476, sqrt_detgammahat_d3xx_volume_element.py:33-130).
+ * L2 = sqrt(integral), RMS = sqrt(integral / volume), 0 when the volume is 0
+ * (diagnostics_volume_integration_helpers.h:332-345). */
+#define BHAH_DIAG_NUM_RECIPES 2
+#define BHAH_DIAG_NUM_INTEGRANDS 3
+typedef struct bhah_volume_integrals {
+  double volume[BHAH_DIAG_NUM_RECIPES];
+  double integral[BHAH_DIAG_NUM_RECIPES][BHAH_DIAG_NUM_INTEGRANDS];
+  double L2[BHAH_DIAG_NUM_RECIPES][BHAH_DIAG_NUM_INTEGRANDS];
+  double RMS[BHAH_DIAG_NUM_RECIPES][BHAH_DIAG_NUM_INTEGRANDS];
+} bhah_volume_integrals;
+
 /* Grid geometry of a created patch (numerical_grid_params_Nxx_dxx_xx). */
 typedef struct bhah_patch_info {
-  int Nxx[3], NGHOSTS, Nxx_plus_2NGHOSTS[3];
+  int Nxx[3], nghosts, Nxx_plus_2NGHOSTS[3]; /* lower-case: a project #defines NGHOSTS */
   int num_evol_gfs, num_auxevol_gfs;
   int64_t Ntot;       /* points per gridfunction, ghost zones included */
   double xxmin[3], xxmax[3], dxx[3], invdxx[3];
@@ -102,7 +132,12 @@ int bhah_hip_abi_version(void);
  * timestep on the current HIP device.  Replaces numerical_grids_and_timestep
  * (nrpy/infrastructures/BHaH/numerical_grids_and_timestep.py:31-760),
  * rfm_precompute (nrpy/infrastructures/BHaH/rfm_precompute.py:84-400) and
- * bcstruct_set_up (nrpy/infrastructures/BHaH/CurviBoundaryConditions/bcstruct_set_up.py:38-792). */
+ * bcstruct_set_up (nrpy/infrastructures/BHaH/CurviBoundaryConditions/bcstruct_set_up.py:38-792).
+ * With desc->setup_on_device the per-point work runs on the device: the reference-metric
+ * arrays, the eigen-coordinate classification of every ghost point with its parity signs,
+ * order-preserving compaction into the inner and pure-outer lists (the same lists, in the
+ * same order, as the host loops), and the CFL min-reduction; only the O(surface) lists and
+ * the 1D arrays come back to the host. */
 int bhah_hip_patch_create(const bhah_patch_desc *desc, bhah_patch **out);
 int bhah_hip_patch_destroy(bhah_patch *patch);
 int bhah_hip_patch_get_info(const bhah_patch *patch, bhah_patch_info *info);
@@ -115,6 +150,10 @@ int bhah_hip_patch_get_xx(const bhah_patch *patch, double *xx0, double *xx1, dou
 int bhah_hip_patch_get_inner_bc(const bhah_patch *patch, int64_t *dstpt, int64_t *srcpt, int8_t *parity10);
 int bhah_hip_patch_get_outer_bc_count(const bhah_patch *patch, int which_gz, int dirn);
 int bhah_hip_patch_get_outer_bc(const bhah_patch *patch, int which_gz, int dirn, int16_t *i0i1i2_face6);
+/* Host copies of the reference-metric precompute arrays (rfm_precompute.py:196-340):
+ * rfm0[k * Nx0 + i0] = d^k f0/dxx0^k (k = 0..3, f0 = r(xx0)), rfm1[k * Nx1 + i1] = d^k f1/dxx1^k
+ * (k = 0..2, f1 = sin(xx1)). */
+int bhah_hip_patch_get_rfm(const bhah_patch *patch, double *rfm0, double *rfm1);
 
 /* rhs_eval (nrpy/infrastructures/BHaH/general_relativity/rhs_eval.py:97-120; call
  * nrpy/examples/two_blackholes_collide.py:294): BSSN right-hand sides at interior points. */
@@ -140,18 +179,20 @@ int bhah_hip_apply_bcs_outerextrap_and_inner(bhah_patch *patch, double *gfs);
 int bhah_hip_apply_bcs_inner_only(bhah_patch *patch, double *gfs);
 
 /* MoL_malloc_intermediate_stage_gfs (MoLtimestepping/MoL_malloc_intermediate_stage_gfs.py:20):
- * allocates y_nplus1_running_total, k_odd, k_even and auxevol on the device; y_n_gfs is
- * allocated too when it is NULL. */
+ * allocates y_nplus1_running_total, k_odd and k_even on the device (zeroed); y_n_gfs and
+ * auxevol_gfs are allocated too when they are NULL. */
 int bhah_hip_MoL_malloc_intermediate_stage_gfs(bhah_patch *patch, bhah_MoL_gridfunctions *gf);
 int bhah_hip_MoL_free(bhah_patch *patch, bhah_MoL_gridfunctions *gf);
 
 /* MoL_step_forward_in_time, RK4 (MoLtimestepping/MoL_step_forward_in_time.py:39-507; frozen
  * MoL_step_forward_in_time_CUDA__MoL_step_forward_in_time__RK4.cu:182-287): one full step of
- * size dt.  *time and *nn follow the reference's update (time = t_0 + (nn - nn_0 + 1) dt). */
-int bhah_hip_MoL_step_forward_in_time(bhah_patch *patch, bhah_MoL_gridfunctions *gf, double dt, double *time);
-/* nsteps steps; with use_graph != 0 the four substeps are captured once in a hipGraph and
- * replayed (no HIP API work per step beyond the graph launch). */
-int bhah_hip_MoL_step_n(bhah_patch *patch, bhah_MoL_gridfunctions *gf, double dt, int nsteps, int use_graph);
+ * size clock->dt, then the reference's bookkeeping
+ * time = t_0 + (nn - nn_0 + 1) * dt; nn++ (MoL_step_forward_in_time.py:473-481). */
+int bhah_hip_MoL_step_forward_in_time(bhah_patch *patch, bhah_MoL_gridfunctions *gf, bhah_clock *clock);
+/* nsteps steps, each followed by the same clock update; with use_graph != 0 the four
+ * substeps are captured once in a hipGraph and replayed (no HIP API work per step beyond the
+ * graph launch). */
+int bhah_hip_MoL_step_n(bhah_patch *patch, bhah_MoL_gridfunctions *gf, bhah_clock *clock, int nsteps, int use_graph);
 
 /* Kernel schedules.  rhs_eval and Ricci_eval each come in several generated
  * schedules (partitions of the outputs into kernels x register budgets); all
@@ -172,16 +213,31 @@ int bhah_hip_patch_set_tile(bhah_patch *patch, int tile_x, int tile_y);
 int bhah_hip_wave_rhs_eval(bhah_patch *patch, const double *in_gfs, double *rhs_gfs);
 int bhah_hip_wave_apply_bcs(bhah_patch *patch, double *gfs);
 
-/* Diagnostics (nrpy/infrastructures/BHaH/diagnostics/diagnostics.py:46-270; constraints
- * nrpy/equations/general_relativity/BSSN_constraints.py:82-157; volume integrals
- * diagnostics_volume_integration_helpers.h:396-476): Hamiltonian constraint H and momentum
- * constraint MU^i and M^2 = gammabar_ij MU^i MU^j e^{4 phi} at interior points into diag (device,
- * 5*Ntot: H, MU0, MU1, MU2, MSQUARED).  If l2_H is
- * not NULL it receives {RMS of H over the proper volume, sqrt(integral of H^2 dV), volume} with
- * dV = sqrt(det ghat) dxx0 dxx1 dxx2 (synchronises the stream).  Needs RbarDD in auxevol_gfs
- * (call bhah_hip_Ricci_eval on in_gfs first). */
+/* Constraint diagnostics: constraints_eval (nrpy/infrastructures/BHaH/general_relativity/
+ * constraints_eval.py; math nrpy/equations/general_relativity/BSSN_constraints.py:82-203) at
+ * interior points into diag (device, BHAH_NUM_DIAG_GFS * Ntot): H, MU^i, M^2 =
+ * gammabar_ij MU^i MU^j e^{4 phi} and |Lambdabar^i - Delta^i| = sqrt(gammabar_ij dL^i dL^j)
+ * (the reference's DIAG_HAMILTONIAN, DIAG_MSQUARED and DIAG_LAMBDA_CONSTRAINT,
+ * diagnostic_gfs_set.py:75-85).  Needs RbarDD in auxevol_gfs (bhah_hip_Ricci_eval on in_gfs
+ * first).  If integrals is not NULL it receives the two recipes' volume integrals
+ * (diagnostics_volume_integration.py:177-207), reduced on the device in a fixed order, so
+ * repeated calls give bitwise equal results; this synchronises the stream. */
 int bhah_hip_diagnostics_constraints(bhah_patch *patch, const double *auxevol_gfs, const double *in_gfs, double *diag,
-                                     double *l2_H);
+                                     bhah_volume_integrals *integrals);
+
+/* diagnostics (nrpy/infrastructures/BHaH/diagnostics/diagnostics.py:46-270) for one grid: on
+ * output steps, fabs(round(time / every) * every - time) < dt / 2 (diagnostics.py:195), it
+ * evaluates Ricci_eval on y_n into auxevol_gfs, the constraint diagnostics into diag and the
+ * volume integrals into *integrals, and sets *output_step = 1; otherwise it does nothing and
+ * sets *output_step = 0.  File output stays with the caller (the reference's
+ * diags_integration_write_row columns are volume, integral, L2, RMS). */
+int bhah_hip_diagnostics(bhah_patch *patch, const bhah_clock *clock, double diagnostics_output_every,
+                         const double *y_n_gfs, double *auxevol_gfs, double *diag, bhah_volume_integrals *integrals,
+                         int *output_step);
+/* Device buffer for the diagnostic gridfunctions (BHAH_NUM_DIAG_GFS * Ntot doubles); the
+ * reference's driver allocates its diagnostic_gfs the same size (diagnostics.py:204-209, 249). */
+int bhah_hip_diagnostic_gfs_malloc(const bhah_patch *patch, double **diag);
+int bhah_hip_diagnostic_gfs_free(double *diag);
 
 /* Checkpoints (write_checkpoint / read_checkpoint, BHaH/tests/write_checkpoint_write_checkpoint__openmp.c:84-117,
  * read_checkpoint_read_checkpoint__openmp.c:64-160).  The reference's file is
@@ -189,8 +245,11 @@ int bhah_hip_diagnostics_constraints(bhah_patch *patch, const double *auxevol_gf
  * int count; int indices[count] (strictly increasing); double data[count * NUM_EVOL_GFS],
  * point-major (data[e * NUM_EVOL_GFS + gf] = y_n[gf * Ntot + indices[e]]).  The structs are
  * project-generated, so the caller writes them; these calls produce and consume the
- * payload.  Pack lists every point; unpack validates like the reference reader and
- * scatters the listed points into y_n (device).  Both synchronise the patch stream. */
+ * payload.  Pack lists every point; unpack validates like the reference reader, zeroes
+ * all of y_n (device) and scatters the listed points into it
+ * (read_checkpoint_read_checkpoint__openmp.c:163-171).  Both synchronise the patch stream.
+ * After unpacking, the caller sets clock.t_0 = clock.time and clock.nn_0 = clock.nn, as the
+ * reference does (read_checkpoint__openmp.c:183-185). */
 int64_t bhah_hip_checkpoint_payload_bytes(const bhah_patch *patch);
 int bhah_hip_checkpoint_pack(bhah_patch *patch, const double *y_n, void *out);
 int bhah_hip_checkpoint_unpack(bhah_patch *patch, const void *payload, int64_t bytes, double *y_n);

@@ -5,7 +5,8 @@ runtime (``csrc/bhah_hip.hip``) are compiled by ``hipcc`` in parallel, then
 linked.  ``python -m nrpy_amd.build [--regen]``; ``--regen`` first re-emits the
 kernels from the SymPy transcription.  Schedules with early-issued loads are
 also compiled to assembly and refused if they can keep more vector-memory
-operations in flight than the 6-bit vmcnt counter tracks (``isa_check``).
+operations in flight than the 6-bit vmcnt counter tracks, or if a DMA-staged box
+can be read before its loads have retired (``isa_check``).
 """
 
 from __future__ import annotations
@@ -66,8 +67,8 @@ def _compile(src: str) -> str:
                            text=True)
         if r.returncode != 0:
             raise RuntimeError(f"hipcc -S failed for {src}:\n{r.stderr[-4000:]}")
-        with open(asm) as f:
-            isa_check.check(f.read(), src)
+        with open(asm) as f, open(src) as g:
+            isa_check.check(f.read(), src, source_text=g.read())
     return obj
 
 

@@ -923,31 +923,21 @@ def _group_bounds(lb: str, k: int, ngroups: int) -> str:
     return LAUNCH_BOUNDS[lb]
 
 # Candidate schedules per build: (kind, partition, launch bounds).
+# Candidate schedules built per workload: the plain one-thread-per-point schedule first (the
+# fallback for input that is not 16-byte aligned), then the tuned workgroup-tiled schedules
+# and their nearest alternatives.  Schedules the autotuner measured slower on every workload
+# are no longer built (their timings stay in the tuned tables under nrpy_amd/autotune/).
 FULL_CANDIDATES = {
-    "rhs": ["p1_lb1", "p2_lb1_sj_lf_pf16_u32_ck6_w2", "p1_lb1_sj_L16x4x4x1_d2_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_xcd",
-            "p1_lb1_sj_L16x4x4x1_d2_dma_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_xcd", "p1_lb1_sj_L16x4x4x1_d4_dma_rb1_xcd",
-            "p1_lb1_sj_L16x4x4x1_d2_dma_rb1_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_xcd",
-            "p1_lb1_sj_L16x4x4x1_d4_dma_rb1_io_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg8_xcd",
-            "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg8_ub_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg12_ub_xcd",
-            "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg12_ub_mi_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg_ub_mi_xcd"],
-    "ricci": ["r1_lb1", "r2_lb1_sj_lf_pf12_zs16", "r1_lb1_sj_L16x4x4x1_g3_d2_xcd", "r1_lb1_sj_L16x4x4x1_d3_dma_xcd",
-              "r1_lb1_sj_L16x4x4x1_d2_dma_xcd", "r1_lb1_sj_L16x4x4x1_d3_dma_rb1_xcd", "r1_lb1_sj_L16x4x4x1_d4_dma_rb1_xcd",
-              "r1_lb1_sj_L16x4x4x1_g3_d2_dma_rb1_xcd", "r1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_xcd",
-              "r1_lb1_sj_L16x4x4x1_d2_dma_io_xcd", "r1_lb1_sj_L16x4x4x1_d2_dma_io_pg2_xcd",
-              "r1_lb1_sj_L16x4x4x1_d2_dma_io_mi_xcd", "r1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_mi_xcd",
+    "rhs": ["p1_lb1", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg12_ub_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg12_ub_mi_xcd",
+            "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg_ub_mi_xcd"],
+    "ricci": ["r1_lb1", "r1_lb1_sj_L16x4x4x1_d2_dma_io_pg2_xcd", "r1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_mi_xcd",
               "r1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg2_mi_xcd"],
 }
-SMALL_CANDIDATES = {"rhs": ["p1_lb1", "p1_lb1_sl", "p3_lb1_sl", "p1_lb1_sl_ln_pf16", "p3_lb1_sl_ln_pf16",
-                            "p3_lb1_sl_ln_pf16_u32", "p3_lb1_sl_ln_pf16_u32_ck8", "p3_lb1_sl_lf_pf16_u32_ck8",
-                            "p1_lb1_sl_lf_pf16_u32", "p1_lb1_sl_lf_pf16_u32_w2", "p1_lb1_sj_L32x8x1x1_d2_dma",
-                            "p1_lb1_sj_L16x8x2x1_d2_dma", "p1_lb1_sj_L16x8x2x1_d3_dma_rb1",
+SMALL_CANDIDATES = {"rhs": ["p1_lb1", "p1_lb1_sl_lf_pf16_u32_w2", "p1_lb1_sj_L16x8x2x1_d3_dma_rb1",
                             "p1_lb1_sj_L16x8x2x1_d3_dma_rb1_pg8", "p1_lb1_sj_L16x8x2x1_d3_dma_rb1_pg8_ub",
                             "p1_lb1_sj_L16x8x2x1_d3_dma_rb1_pg8_ub_mi"],
-                    "ricci": ["r1_lb1", "r1_lb1_sl", "r2_lb1_sl", "r1_lb1_pf16", "r2_lb1_sl_pf16", "r1_lb1_pf16_u32",
-                              "r2_lb1_sl_pf16_u32", "r2_lb1_sl_pf16_u32_ck8", "r2_lb1_sl_lf_pf16_u32",
-                              "r2_lb1_sl_lf_pf16_u32_w2", "r2_lb1_sj_L16x4x4x1", "r1_lb1_sj_L16x4x4x1_d2_dma_xcd",
-                              "r1_lb1_sj_L32x8x1x1_d2_dma", "r1_lb1_sj_L16x8x2x1_d2_dma", "r1_lb1_sj_L16x8x2x1_d3_dma_rb1",
-                              "r1_lb1_sj_L16x8x2x1_d3_dma_rb1_mi"]}
+                    "ricci": ["r1_lb1", "r1_lb1_pf16_u32", "r1_lb1_sj_L16x4x4x1_d2_dma_xcd",
+                              "r1_lb1_sj_L16x8x2x1_d3_dma_rb1", "r1_lb1_sj_L16x8x2x1_d3_dma_rb1_mi"]}
 
 CAND_SIG = ("dim3 g, dim3 b, hipStream_t s, const double *in, const double *aux, double *out, "
             "const double *rfm0, const double *rfm1, bhah_kparams p")
@@ -1009,6 +999,9 @@ LDS_DMA_MACROS = """#ifndef BHAH_DMA16
 #define BHAH_VMWAIT(n) __builtin_amdgcn_s_waitcnt(((n) & 15) | (((n) >> 4) << 14) | 0x0F70)
 // workgroup barrier without a memory fence (LDS reads of this wave retired first)
 #define BHAH_BARRIER() asm volatile("s_waitcnt lgkmcnt(0)\\n\\ts_barrier" ::: "memory")
+// the same before the LDS reads of DMA phase n; the comment marks it in the assembly listing
+// for the build's wait check (nrpy_amd/isa_check.py)
+#define BHAH_BARRIER_PHASE(n) asm volatile("s_waitcnt lgkmcnt(0)\\n\\ts_barrier ; bhah_read_phase " #n ::: "memory")
 #endif
 """
 
@@ -1301,7 +1294,10 @@ def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str],
         # loads issued after a phase's own is known at compile time: waiting until at most
         # that many are outstanding means this phase's boxes have landed (vmcnt counts in
         # order); the barrier then publishes all waves' boxes
+        issue: List[Tuple[int, int]] = []  # (phase, loads per lane) in issue order
+
         def dma_phase(ph):
+            issue.append((ph, len(phases[ph]) * NJ))
             for q, g in enumerate(phases[ph]):
                 a(f"""  {{
     const char *src = (const char *)(in + {sub.gfs.index(g)} * Ntot);
@@ -1323,7 +1319,7 @@ def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str],
                     raise ValueError("DMA prefetch deeper than the 6-bit vmcnt counter")
                 a(f"  // ---- {', '.join(gfs)}")
                 a(f"  BHAH_VMWAIT({later});")
-                a("  BHAH_BARRIER();")
+                a(f"  BHAH_BARRIER_PHASE({ph});")
                 if ph + depth - 1 < len(phases):
                     dma_phase(ph + depth - 1)
                 emit_phase(ph, gfs)
@@ -1336,7 +1332,7 @@ def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str],
                     raise ValueError("DMA prefetch deeper than the 6-bit vmcnt counter")
                 a(f"  // ---- {', '.join(gfs)}")
                 a(f"  BHAH_VMWAIT({later});")
-                a("  BHAH_BARRIER();")
+                a(f"  BHAH_BARRIER_PHASE({ph});")
                 emit_phase(ph, gfs)
                 if ph + depth < len(phases):
                     a("  BHAH_BARRIER();")  # every wave is done with the ring slot about to be refilled
@@ -1381,8 +1377,13 @@ def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str],
   }}""")
     tile_body = "\n".join(L)
     bounds = f"{NT}{f', {min_waves}' if min_waves > 1 else ''}"
+    dma_note = ""
+    if dma:
+        # read by nrpy_amd/isa_check.py: DMA groups (phase:loads per lane) in issue order
+        groups = " ".join(f"{ph}:{n}" for ph, n in issue)
+        dma_note = "".join(f"// bhah-dma-issue {k}: {groups}\n" for k in [kname] + ([ensemble] if ensemble else []))
     src = f"""
-extern "C" __global__ void __launch_bounds__({bounds})
+{dma_note}extern "C" __global__ void __launch_bounds__({bounds})
 {kname}(const double *__restrict__ in, const double *__restrict__ aux, double *__restrict__ out,
         const double *__restrict__ rfm0, const double *__restrict__ rfm1, const bhah_kparams p) {{
   __shared__ double sh[{nslots * SLOT}];

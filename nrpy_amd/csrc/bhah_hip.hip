@@ -12,6 +12,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -101,14 +102,14 @@ struct bhah_patch {
   std::vector<int8_t> inner_parity;  // 10 per point
   std::vector<int16_t> outer[8][3];  // 6 shorts per point
   // device arrays
-  double *d_rfm0 = nullptr, *d_rfm1 = nullptr, *d_xx0 = nullptr;
+  double *d_rfm0 = nullptr, *d_rfm1 = nullptr, *d_xx0 = nullptr, *d_xx1 = nullptr, *d_xx2 = nullptr;
   int *d_inner_dst = nullptr, *d_inner_src = nullptr;
   unsigned short *d_inner_mask = nullptr;  // bit t set: parity type t flips sign
   int16_t *d_outer[8][3] = {};
   int *d_rad_lines = nullptr;  // (i1, i2) per radial line when the outer points form lines
   int n_rad_lines = 0;
   double *d_wavespeed = nullptr, *d_finf = nullptr;
-  double *d_red = nullptr;
+  double *d_red = nullptr;  // volume-integral partial sums (VI_MAX_BLOCKS x VI_SUMS) and totals
   int *d_gf_parity = nullptr;
   hipStream_t stream = nullptr;
   bool own_stream = false;
@@ -520,6 +521,13 @@ rk_combine_kernel(double *__restrict__ out, const double *__restrict__ y, const 
   out[t] = y[t] + c0 * k[t] + c1 * k[n + t] + c2 * k[2 * n + t] + c3 * k[3 * n + t];
 }
 
+// dst[0..n) = v
+__global__ void __launch_bounds__(256)
+fill_kernel(double *__restrict__ dst, double v, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) dst[t] = v;
+}
+
 // dst += a * src
 __global__ void __launch_bounds__(256)
 axpy_kernel(double *__restrict__ dst, const double *__restrict__ src, double a, long n) {
@@ -581,41 +589,274 @@ initial_data_bl_kernel(double *__restrict__ y, int coord, double AMPL, double SI
   for (int g = 0; g < BHAH_NUM_EVOL_GFS_BSSN; g++) y[g * ntot + p] = (g == 6 || g == 10) ? w : 0.0;
 }
 
-// Volume-weighted sums over the interior for the constraint norms.
+// ---- Volume integrals of the diagnostics recipes (whole domain; |x| > 8), reduced in a
+// fixed order so that every call gives bitwise the same sums: each workgroup of the first
+// pass walks interior points in array order with a grid stride and writes its eight partial
+// sums (per recipe: volume, integral of H^2, of MSQUARED^2, of LAMBDA_CONSTRAINT^2); the
+// second pass adds the partials of all workgroups in workgroup order.
+constexpr int VI_SUMS = 4 * BHAH_DIAG_NUM_RECIPES;
+constexpr int VI_MAX_BLOCKS = 1024;
+constexpr double VI_SPHERE_RADIUS = 8.0;  // diagnostics_volume_integration.py:193
+
 __global__ void __launch_bounds__(256)
-l2_reduce_kernel(const double *__restrict__ h, const double *__restrict__ msq, const double *__restrict__ rfm0,
-                 const double *__restrict__ rfm1, bhah_kparams p, double dvol, double *__restrict__ out) {
-  __shared__ double sh[3][256];
-  const int i0 = p.NG + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int i1 = p.NG + (int)(blockIdx.y * blockDim.y + threadIdx.y);
-  const int i2 = p.NG + (int)blockIdx.z;
-  double a = 0, b = 0, c = 0;
-  if (i0 < p.Nx0 - p.NG && i1 < p.Nx1 - p.NG) {
+volume_integrals_partial_kernel(const double *__restrict__ diag, const double *__restrict__ rfm0,
+                                const double *__restrict__ rfm1, const double *__restrict__ xx1,
+                                const double *__restrict__ xx2, bhah_kparams p, double dxx012,
+                                double *__restrict__ partial) {
+  __shared__ double sh[VI_SUMS][256];
+  const int n0 = p.Nx0 - 2 * p.NG, n1 = p.Nx1 - 2 * p.NG, n2 = p.Nx2 - 2 * p.NG;
+  const long nint = (long)n0 * n1 * n2;
+  double acc[VI_SUMS];
+#pragma unroll
+  for (int k = 0; k < VI_SUMS; k++) acc[k] = 0.0;
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < nint; q += (long)gridDim.x * blockDim.x) {
+    const int i0 = p.NG + (int)(q % n0), i1 = p.NG + (int)((q / n0) % n1), i2 = p.NG + (int)(q / ((long)n0 * n1));
     const long idx = i0 + p.s1 * i1 + p.s2 * i2;
-    const double f0 = rfm0[i0], f0p = rfm0[p.Nx0 + i0], f1 = rfm1[i1];
-    const double dV = fabs(f0p * f0 * f0 * f1) * dvol;
-    a = dV;
-    b = h[idx] * h[idx] * dV;
-    c = msq[idx] * dV;
+    // sqrt(det gammahat) |dxx0 dxx1 dxx2| of spherical-like coordinates: |f0' f0^2 sin(xx1)| d^3xx
+    const double f0 = rfm0[i0], f0p = rfm0[p.Nx0 + i0];
+    const double dV = fabs(f0p * f0 * f0 * rfm1[i1]) * dxx012;
+    if (dV == 0.0) continue;
+    const double H = diag[idx], M2 = diag[BHAH_DIAG_MSQUARED * p.Ntot + idx],
+                 L = diag[BHAH_DIAG_LAMBDA_CONSTRAINT * p.Ntot + idx];
+    const double v[4] = {dV, H * H * dV, M2 * M2 * dV, L * L * dV};
+#pragma unroll
+    for (int k = 0; k < 4; k++) acc[k] += v[k];
+    // recipe 1 keeps points outside the sphere |x| <= 8 (exclude_inside = 1)
+    const double X = f0 * sin(xx1[i1]) * cos(xx2[i2]), Y = f0 * sin(xx1[i1]) * sin(xx2[i2]), Z = f0 * cos(xx1[i1]);
+    if (!(sqrt(X * X + Y * Y + Z * Z) <= VI_SPHERE_RADIUS)) {
+#pragma unroll
+      for (int k = 0; k < 4; k++) acc[4 + k] += v[k];
+    }
   }
-  const int tid = threadIdx.x + blockDim.x * threadIdx.y;
-  sh[0][tid] = a;
-  sh[1][tid] = b;
-  sh[2][tid] = c;
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < VI_SUMS; k++) sh[k][tid] = acc[k];
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
     if (tid < s) {
-      sh[0][tid] += sh[0][tid + s];
-      sh[1][tid] += sh[1][tid + s];
-      sh[2][tid] += sh[2][tid + s];
+#pragma unroll
+      for (int k = 0; k < VI_SUMS; k++) sh[k][tid] += sh[k][tid + s];
     }
     __syncthreads();
   }
-  if (tid == 0) {
-    atomicAdd(&out[0], sh[0][0]);
-    atomicAdd(&out[1], sh[1][0]);
-    atomicAdd(&out[2], sh[2][0]);
+  if (tid < VI_SUMS) partial[(long)blockIdx.x * VI_SUMS + tid] = sh[tid][0];
+}
+
+__global__ void __launch_bounds__(64)
+volume_integrals_final_kernel(const double *__restrict__ partial, int nblocks, double *__restrict__ out) {
+  const int k = threadIdx.x;
+  if (k >= VI_SUMS) return;
+  double s = 0.0;
+  for (int b = 0; b < nblocks; b++) s += partial[(long)b * VI_SUMS + k];
+  out[k] = s;
+}
+
+// ---- Device-side set-up (bhah_hip_patch_create_on_device): reference-metric arrays, the
+// classification of ghost points into inner and pure outer boundary points, order-preserving
+// compaction of both lists, and the CFL reduction, with the host set-up's arithmetic
+// (contraction into fused multiply-adds is off, so coordinates and index maps agree bit for
+// bit; exp, sin, cos, acos and atan2 come from the device math library).
+struct SetupGeom {
+  int coord, NG, Nx0, Nx1, Nx2;
+  long Ntot;
+  double xxmin[3], dxx[3], AMPL, SINHW;
+};
+struct SetupBox {
+  int lo[3], hi[3];  // points lo <= i < hi, enumerated with i0 fastest (the host loops' order)
+  int face[3];       // outward face normal recorded with pure outer points
+};
+constexpr int SETUP_BLOCK = 256;
+
+__device__ __forceinline__ double setup_xx(const SetupGeom &g, int d, int j) {
+#pragma clang fp contract(off)
+  return g.xxmin[d] + ((double)(j - g.NG) + (1.0 / 2.0)) * g.dxx[d];
+}
+
+__global__ void __launch_bounds__(256) setup_rfm_kernel(SetupGeom g, double *__restrict__ rfm0,
+                                                        double *__restrict__ rfm1) {
+#pragma clang fp contract(off)
+  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (t < g.Nx0) {
+    const double x = setup_xx(g, 0, t);
+    double f = 1, d1 = 0, d2 = 0, d3 = 0;
+    if (g.coord == BHAH_SPHERICAL) {
+      f = x;
+      d1 = 1.0;
+    } else if (g.coord == BHAH_SINHSPHERICAL) {
+      const double A = g.AMPL, w = g.SINHW;
+      const double den = exp(1.0 / w) - exp(-1.0 / w);
+      const double ep = exp(x / w), em = exp(-x / w);
+      f = A * (ep - em) / den;
+      d1 = A * (ep + em) / (w * den);
+      d2 = A * (ep - em) / (w * w * den);
+      d3 = A * (ep + em) / (w * w * w * den);
+    }
+    rfm0[t] = f;
+    rfm0[g.Nx0 + t] = d1;
+    rfm0[2 * g.Nx0 + t] = d2;
+    rfm0[3 * g.Nx0 + t] = d3;
   }
+  if (t < g.Nx1) {
+    const double y = setup_xx(g, 1, t);
+    const bool cart = g.coord == BHAH_CARTESIAN;
+    rfm1[t] = cart ? 1.0 : sin(y);
+    rfm1[g.Nx1 + t] = cart ? 0.0 : cos(y);
+    rfm1[2 * g.Nx1 + t] = cart ? 0.0 : -sin(y);
+  }
+}
+
+// kind[p]: 0 interior, 1 inner boundary point (src[p], mask[p] set), 2 pure outer point.
+// err bit 1: an inner point maps outside the grid; bit 2: a parity dot product is not +-1.
+__global__ void __launch_bounds__(256) setup_classify_kernel(SetupGeom g, unsigned char *__restrict__ kind,
+                                                             int *__restrict__ src, unsigned short *__restrict__ mask,
+                                                             int *__restrict__ err) {
+#pragma clang fp contract(off)
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.Ntot) return;
+  const int i0 = (int)(p % g.Nx0), i1 = (int)((p / g.Nx0) % g.Nx1), i2 = (int)(p / ((long)g.Nx0 * g.Nx1));
+  const int NG = g.NG;
+  if (i0 >= NG && i0 < g.Nx0 - NG && i1 >= NG && i1 < g.Nx1 - NG && i2 >= NG && i2 < g.Nx2 - NG) {
+    kind[p] = 0;
+    return;
+  }
+  if (g.coord == BHAH_CARTESIAN) {
+    kind[p] = 2;
+    return;
+  }
+  // spherical eigen-coordinate double map xx -> Cart -> xx' (as eigen_inbounds on the host)
+  const int Nx[3] = {g.Nx0, g.Nx1, g.Nx2};
+  const double r = setup_xx(g, 0, i0), th = setup_xx(g, 1, i1), ph = setup_xx(g, 2, i2);
+  const double X = r * sin(th) * cos(ph), Y = r * sin(th) * sin(ph), Z = r * cos(th);
+  const double rr = sqrt(X * X + Y * Y + Z * Z);
+  const double c[3] = {rr, acos(Z / rr), atan2(Y, X)};
+  int ib[3];
+  for (int d = 0; d < 3; d++) {
+    const double xxmin = setup_xx(g, d, 0) - ((double)(0 - NG) + (1.0 / 2.0)) * g.dxx[d];
+    ib[d] = (int)((c[d] - xxmin - (1.0 / 2.0) * g.dxx[d] + ((double)NG) * g.dxx[d]) / g.dxx[d] + 0.5);
+  }
+  if (ib[0] == i0 && ib[1] == i1 && ib[2] == i2) {
+    kind[p] = 2;
+    return;
+  }
+  for (int d = 0; d < 3; d++)
+    if (ib[d] < 0 || ib[d] >= Nx[d]) {
+      atomicOr(err, 1);
+      kind[p] = 3;
+      return;
+    }
+  const double xin1 = setup_xx(g, 1, ib[1]), xin2 = setup_xx(g, 2, ib[2]);
+  const double t0 = cos(th) * cos(xin1);
+  const double t1 = sin(th) * sin(xin1);
+  const double t2 = sin(ph) * sin(xin2);
+  const double t3 = cos(ph) * cos(xin2);
+  const double p1 = t0 + t1 * t2 + t1 * t3;
+  const double p2 = t0 * t2 + t0 * t3 + t1;
+  const double p3 = t2 + t3;
+  const double dots[10] = {1, p1, p2, p3, p1 * p1, p1 * p2, p1 * p3, p2 * p2, p2 * p3, p3 * p3};
+  unsigned m = 0;
+  for (int k = 0; k < 10; k++) {
+    if (fabs(dots[k]) < 1 - 1e-8 || fabs(dots[k]) > 1 + 1e-8) atomicOr(err, 2);
+    if (dots[k] < 0) m |= 1u << k;
+  }
+  kind[p] = 1;
+  src[p] = ib[0] + Nx[0] * (ib[1] + Nx[1] * ib[2]);
+  mask[p] = (unsigned short)m;
+}
+
+__device__ __forceinline__ long setup_box_point(const SetupGeom &g, const SetupBox &b, long q, int &i0, int &i1,
+                                                int &i2) {
+  const int n0 = b.hi[0] - b.lo[0], n1 = b.hi[1] - b.lo[1];
+  i0 = b.lo[0] + (int)(q % n0);
+  i1 = b.lo[1] + (int)((q / n0) % n1);
+  i2 = b.lo[2] + (int)(q / ((long)n0 * n1));
+  return i0 + (long)g.Nx0 * (i1 + (long)g.Nx1 * i2);
+}
+
+// Points of kind `want` per block of SETUP_BLOCK consecutive box points.
+__global__ void __launch_bounds__(SETUP_BLOCK) setup_count_kernel(SetupGeom g, SetupBox b, long n,
+                                                                  const unsigned char *__restrict__ kind, int want,
+                                                                  int *__restrict__ counts) {
+  __shared__ int wc[SETUP_BLOCK / 64];
+  const long q = (long)blockIdx.x * SETUP_BLOCK + threadIdx.x;
+  int i0, i1, i2;
+  const bool f = q < n && kind[setup_box_point(g, b, q < n ? q : 0, i0, i1, i2)] == want;
+  const unsigned long long m = __ballot(f);
+  if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = __popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int s = 0;
+    for (int w = 0; w < SETUP_BLOCK / 64; w++) s += wc[w];
+    counts[blockIdx.x] = s;
+  }
+}
+
+// Write the selected points in box order from each block's offset: inner points as
+// (dst, src, mask), pure outer points as (i0, i1, i2, face normal).
+__global__ void __launch_bounds__(SETUP_BLOCK) setup_scatter_kernel(
+    SetupGeom g, SetupBox b, long n, const unsigned char *__restrict__ kind, int want, const int *__restrict__ offsets,
+    const int *__restrict__ src, const unsigned short *__restrict__ mask, int *__restrict__ o_dst,
+    int *__restrict__ o_src, unsigned short *__restrict__ o_mask, int16_t *__restrict__ o_outer) {
+  __shared__ int wc[SETUP_BLOCK / 64];
+  const long q = (long)blockIdx.x * SETUP_BLOCK + threadIdx.x;
+  int i0 = 0, i1 = 0, i2 = 0;
+  const long p = setup_box_point(g, b, q < n ? q : 0, i0, i1, i2);
+  const bool f = q < n && kind[p] == want;
+  const unsigned long long m = __ballot(f);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) wc[wave] = __popcll(m);
+  __syncthreads();
+  if (!f) return;
+  int pos = offsets[blockIdx.x];
+  for (int w = 0; w < wave; w++) pos += wc[w];
+  pos += __popcll(m & ((1ull << lane) - 1ull));
+  if (want == 1) {
+    o_dst[pos] = (int)p;
+    o_src[pos] = src[p];
+    o_mask[pos] = mask[p];
+  } else {
+    int16_t *o = &o_outer[6 * (long)pos];
+    o[0] = (int16_t)i0;
+    o[1] = (int16_t)i1;
+    o[2] = (int16_t)i2;
+    o[3] = (int16_t)b.face[0];
+    o[4] = (int16_t)b.face[1];
+    o[5] = (int16_t)b.face[2];
+  }
+}
+
+// Per-block minimum over all points of min(ds0, ds1, ds2) (the CFL condition).
+__global__ void __launch_bounds__(256) setup_cfl_kernel(SetupGeom g, double *__restrict__ partial) {
+#pragma clang fp contract(off)
+  __shared__ double sh[256];
+  double mn = 1e38;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < g.Ntot; p += (long)gridDim.x * blockDim.x) {
+    const int i0 = (int)(p % g.Nx0), i1 = (int)((p / g.Nx0) % g.Nx1);
+    const double x0 = setup_xx(g, 0, i0), x1 = setup_xx(g, 1, i1);
+    double ds0, ds1, ds2;
+    if (g.coord == BHAH_SPHERICAL) {
+      ds0 = fabs(g.dxx[0]);
+      ds1 = fabs(x0 * g.dxx[1]);
+      ds2 = fabs(x0 * g.dxx[2] * sin(x1));
+    } else {
+      const double tmp0 = (1.0 / (g.SINHW));
+      const double tmp4 = g.AMPL / (exp(tmp0) - exp(-tmp0));
+      const double tmp2 = exp(tmp0 * x0);
+      const double tmp3 = exp(-tmp0 * x0);
+      const double tmp5 = tmp4 * (tmp2 - tmp3);
+      ds0 = fabs(g.dxx[0] * tmp4 * (tmp0 * tmp2 + tmp0 * tmp3));
+      ds1 = fabs(g.dxx[1] * tmp5);
+      ds2 = fabs(g.dxx[2] * tmp5 * sin(x1));
+    }
+    const double m = fmin(ds0, fmin(ds1, ds2));
+    mn = m < mn ? m : mn;
+  }
+  sh[threadIdx.x] = mn;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] = fmin(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -773,6 +1014,35 @@ int check_ptr(const void *p, const char *name) {
   return BHAH_OK;
 }
 
+// Two-pass volume integrals of the diagnostic gridfunctions diag: the first pass writes
+// VI_SUMS partial sums per workgroup into partial (VI_MAX_BLOCKS * VI_SUMS doubles), the
+// second their totals into out (VI_SUMS doubles).  Every partial a call reads is written by
+// the same call, so the buffers need no clearing.
+int enqueue_volume_integrals(bhah_patch *P, const double *diag, double *partial, double *out) {
+  const long nint = (long)P->d.Nxx[0] * P->d.Nxx[1] * P->d.Nxx[2];
+  const int nb = (int)std::min<long>(VI_MAX_BLOCKS, std::max<long>(1, (nint + 255) / 256));
+  hipLaunchKernelGGL(volume_integrals_partial_kernel, dim3(nb), dim3(256), 0, P->stream, diag, P->d_rfm0, P->d_rfm1,
+                     P->d_xx1, P->d_xx2, P->kp(0, 0), fabs(P->dxx[0] * P->dxx[1] * P->dxx[2]), partial);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(volume_integrals_final_kernel, dim3(1), dim3(64), 0, P->stream, partial, nb, out);
+  HIP_TRY(hipGetLastError());
+  return BHAH_OK;
+}
+
+// L2 and RMS columns of diags_integration_write_row (diagnostics_volume_integration_helpers.h:332-345).
+void finish_volume_integrals(const double sums[VI_SUMS], bhah_volume_integrals *r) {
+  for (int q = 0; q < BHAH_DIAG_NUM_RECIPES; q++) {
+    const double vol = sums[4 * q];
+    r->volume[q] = vol;
+    for (int k = 0; k < BHAH_DIAG_NUM_INTEGRANDS; k++) {
+      const double I = sums[4 * q + 1 + k];
+      r->integral[q][k] = I;
+      r->L2[q][k] = sqrt(fmax(I, 0.0));
+      r->RMS[q][k] = vol > 0.0 ? sqrt(fmax(I, 0.0) / vol) : 0.0;
+    }
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -782,7 +1052,17 @@ extern "C" {
 const char *bhah_hip_last_error(void) { return g_last_error.c_str(); }
 int bhah_hip_abi_version(void) { return BHAH_HIP_ABI_VERSION; }
 
+static int setup_on_host(bhah_patch &P, std::vector<double> &rfm0, std::vector<double> &rfm1);
+static int setup_on_device(bhah_patch &P, std::vector<double> &rfm0, std::vector<double> &rfm1);
+static int patch_upload(bhah_patch *P, const std::vector<double> &rfm0, const std::vector<double> &rfm1,
+                        bhah_patch **out);
+static int patch_create_impl(const bhah_patch_desc *desc, bhah_patch **out, bool on_device);
+
 int bhah_hip_patch_create(const bhah_patch_desc *desc, bhah_patch **out) {
+  return patch_create_impl(desc, out, desc && desc->setup_on_device);
+}
+
+static int patch_create_impl(const bhah_patch_desc *desc, bhah_patch **out, bool on_device) {
   if (!desc || !out) return fail(BHAH_ERR_ARG, "NULL argument");
   *out = nullptr;
   const bhah_patch_desc &d = *desc;
@@ -841,9 +1121,21 @@ int bhah_hip_patch_create(const bhah_patch_desc *desc, bhah_patch **out) {
     delete P;
     return fail(BHAH_ERR_ARG, "grid too large");
   }
+  std::vector<double> rfm0(4 * P->Nx[0]), rfm1(3 * P->Nx[1]);
+  int rc = on_device ? setup_on_device(*P, rfm0, rfm1) : setup_on_host(*P, rfm0, rfm1);
+  if (rc) {
+    delete P;
+    return rc;
+  }
+  return patch_upload(P, rfm0, rfm1, out);
+}
+
+// Reference-metric arrays, boundary structs and CFL timestep on the host.
+static int setup_on_host(bhah_patch &PP, std::vector<double> &rfm0, std::vector<double> &rfm1) {
+  bhah_patch *P = &PP;
+  const bhah_patch_desc &d = P->d;
   // rfm precompute arrays: f0, f0', f0'', f0''' over xx0 and f1, f1', f1'' over xx1
   // (rfm_precompute.py:196-340 with the closed forms of reference_metric.py:1338-1357).
-  std::vector<double> rfm0(4 * P->Nx[0]), rfm1(3 * P->Nx[1]);
   for (int i = 0; i < P->Nx[0]; i++) {
     const double x = P->xx[0][i];
     double f = 1, d1 = 0, d2 = 0, d3 = 0;
@@ -872,10 +1164,7 @@ int bhah_hip_patch_create(const bhah_patch_desc *desc, bhah_patch **out) {
     rfm1[2 * P->Nx[1] + j] = cart ? 0.0 : -sin(y);
   }
   int rc = build_bcstruct(*P);
-  if (rc) {
-    delete P;
-    return rc;
-  }
+  if (rc) return rc;
   // CFL timestep: dt = CFL * min over all points of min_i ds_i
   // (numerical_grids_and_timestep_cfl_limited_timestep__openmp.c; ds_min_single_pt…HoleySinhSpherical.c).
   double ds_min = 1e38;
@@ -906,7 +1195,176 @@ int bhah_hip_patch_create(const bhah_patch_desc *desc, bhah_patch **out) {
         }
   }
   P->dt = d.CFL_FACTOR * ds_min;
-  // device data
+  return BHAH_OK;
+}
+
+// Device buffers of the set-up, freed when it returns.
+struct SetupTmp {
+  std::vector<void *> ptrs;
+  ~SetupTmp() {
+    for (void *q : ptrs) (void)hipFree(q);
+  }
+  int alloc(void **out, size_t bytes) {
+    *out = nullptr;
+    if (hipMalloc(out, bytes ? bytes : 1) != hipSuccess)
+      return fail(BHAH_ERR_HIP, "hipMalloc of the set-up workspace failed");
+    ptrs.push_back(*out);
+    return BHAH_OK;
+  }
+};
+#define SETUP_ALLOC(t, ptr, n) (t).alloc((void **)&(ptr), sizeof(*(ptr)) * (size_t)(n))
+
+// The points of kind `want` in box b, in box order, appended to the host lists: inner points
+// to (dst, src, mask), pure outer points to outer (6 shorts each).
+static int setup_select(const SetupGeom &g, const SetupBox &b, const unsigned char *kind, int want, const int *src,
+                        const unsigned short *mask, std::vector<int> &dst_h, std::vector<int> &src_h,
+                        std::vector<unsigned short> &mask_h, std::vector<int16_t> &outer_h) {
+  const long n = (long)(b.hi[0] - b.lo[0]) * (b.hi[1] - b.lo[1]) * (b.hi[2] - b.lo[2]);
+  if (n <= 0) return BHAH_OK;
+  const int nb = (int)((n + SETUP_BLOCK - 1) / SETUP_BLOCK);
+  SetupTmp t;
+  int rc, *counts, *offs;
+  if ((rc = SETUP_ALLOC(t, counts, nb)) || (rc = SETUP_ALLOC(t, offs, nb))) return rc;
+  hipLaunchKernelGGL(setup_count_kernel, dim3(nb), dim3(SETUP_BLOCK), 0, nullptr, g, b, n, kind, want, counts);
+  HIP_TRY(hipGetLastError());
+  std::vector<int> h(nb);
+  HIP_TRY(hipMemcpy(h.data(), counts, sizeof(int) * nb, hipMemcpyDeviceToHost));
+  long total = 0;
+  for (int i = 0; i < nb; i++) {
+    const int c = h[i];
+    h[i] = (int)total;
+    total += c;
+  }
+  if (!total) return BHAH_OK;
+  HIP_TRY(hipMemcpy(offs, h.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
+  int *o_dst = nullptr, *o_src = nullptr;
+  unsigned short *o_mask = nullptr;
+  int16_t *o_outer = nullptr;
+  if (want == 1) {
+    if ((rc = SETUP_ALLOC(t, o_dst, total)) || (rc = SETUP_ALLOC(t, o_src, total)) || (rc = SETUP_ALLOC(t, o_mask, total))) return rc;
+  } else if ((rc = SETUP_ALLOC(t, o_outer, 6 * total))) {
+    return rc;
+  }
+  hipLaunchKernelGGL(setup_scatter_kernel, dim3(nb), dim3(SETUP_BLOCK), 0, nullptr, g, b, n, kind, want, offs, src,
+                     mask, o_dst, o_src, o_mask, o_outer);
+  HIP_TRY(hipGetLastError());
+  if (want == 1) {
+    const size_t s = dst_h.size();
+    dst_h.resize(s + total);
+    src_h.resize(s + total);
+    mask_h.resize(s + total);
+    HIP_TRY(hipMemcpy(dst_h.data() + s, o_dst, sizeof(int) * total, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(src_h.data() + s, o_src, sizeof(int) * total, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(mask_h.data() + s, o_mask, sizeof(unsigned short) * total, hipMemcpyDeviceToHost));
+  } else {
+    const size_t s = outer_h.size();
+    outer_h.resize(s + 6 * total);
+    HIP_TRY(hipMemcpy(outer_h.data() + s, o_outer, sizeof(int16_t) * 6 * total, hipMemcpyDeviceToHost));
+  }
+  return BHAH_OK;
+}
+
+// Reference-metric arrays, boundary structs and CFL timestep computed on the device; the
+// boundary lists (O(surface)) and the 1D arrays come back to the host copies the patch keeps.
+static int setup_on_device(bhah_patch &P, std::vector<double> &rfm0, std::vector<double> &rfm1) {
+  const bhah_patch_desc &d = P.d;
+  SetupGeom g;
+  g.coord = d.coord;
+  g.NG = P.NG;
+  g.Nx0 = P.Nx[0];
+  g.Nx1 = P.Nx[1];
+  g.Nx2 = P.Nx[2];
+  g.Ntot = P.Ntot;
+  for (int i = 0; i < 3; i++) {
+    g.xxmin[i] = P.xxmin[i];
+    g.dxx[i] = P.dxx[i];
+  }
+  g.AMPL = d.grid_physical_size;
+  g.SINHW = d.SINHW;
+  SetupTmp t;
+  int rc;
+  double *d_r0, *d_r1, *partial;
+  unsigned char *kind;
+  int *src, *err;
+  unsigned short *mask;
+  if ((rc = SETUP_ALLOC(t, d_r0, rfm0.size())) || (rc = SETUP_ALLOC(t, d_r1, rfm1.size())) || (rc = SETUP_ALLOC(t, kind, P.Ntot)) ||
+      (rc = SETUP_ALLOC(t, src, P.Ntot)) || (rc = SETUP_ALLOC(t, mask, P.Ntot)) || (rc = SETUP_ALLOC(t, err, 1)) ||
+      (rc = SETUP_ALLOC(t, partial, VI_MAX_BLOCKS)))
+    return rc;
+  const int n1d = std::max(P.Nx[0], P.Nx[1]);
+  hipLaunchKernelGGL(setup_rfm_kernel, dim3((n1d + 255) / 256), dim3(256), 0, nullptr, g, d_r0, d_r1);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpy(rfm0.data(), d_r0, sizeof(double) * rfm0.size(), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(rfm1.data(), d_r1, sizeof(double) * rfm1.size(), hipMemcpyDeviceToHost));
+  // classification of every ghost point, then the inner list in index order
+  HIP_TRY(hipMemset(err, 0, sizeof(int)));
+  hipLaunchKernelGGL(setup_classify_kernel, dim3((unsigned)((P.Ntot + 255) / 256)), dim3(256), 0, nullptr, g, kind,
+                     src, mask, err);
+  HIP_TRY(hipGetLastError());
+  int herr = 0;
+  HIP_TRY(hipMemcpy(&herr, err, sizeof(int), hipMemcpyDeviceToHost));
+  if (herr & 1) return fail(BHAH_ERR_BCSTRUCT, "an inner boundary point maps outside the grid");
+  if (herr & 2) return fail(BHAH_ERR_BCSTRUCT, "a parity dot product is not +-1");
+  std::vector<int> dst_h, src_h;
+  std::vector<unsigned short> mask_h;
+  std::vector<int16_t> none;
+  const SetupBox all = {{0, 0, 0}, {P.Nx[0], P.Nx[1], P.Nx[2]}, {0, 0, 0}};
+  if ((rc = setup_select(g, all, kind, 1, src, mask, dst_h, src_h, mask_h, none))) return rc;
+  for (size_t q = 0; q < dst_h.size(); q++) {
+    P.inner_dst.push_back(dst_h[q]);
+    P.inner_src.push_back(src_h[q]);
+    for (int k = 0; k < 10; k++) P.inner_parity.push_back(((mask_h[q] >> k) & 1u) ? -1 : 1);
+  }
+  // pure outer points, face by face in the order of build_bcstruct (innermost layer first;
+  // each face spans the current bounds, then widens its own direction)
+  const int NG = P.NG;
+  int imin[3] = {NG, NG, NG}, imax[3] = {P.Nx[0] - NG, P.Nx[1] - NG, P.Nx[2] - NG};
+  for (int gz = 0; gz < NG; gz++)
+    for (int dd = 0; dd < 3; dd++)
+      for (int side = 0; side < 2; side++) {
+        SetupBox b;
+        for (int e = 0; e < 3; e++) {
+          b.lo[e] = imin[e];
+          b.hi[e] = imax[e];
+        }
+        if (side == 0) {
+          b.lo[dd] = imin[dd] - 1;
+          b.hi[dd] = imin[dd];
+          imin[dd]--;
+        } else {
+          b.lo[dd] = imax[dd];
+          b.hi[dd] = imax[dd] + 1;
+          imax[dd]++;
+        }
+        const int face = 2 * dd + side;
+        b.face[0] = (face == 0) - (face == 1);
+        b.face[1] = (face == 2) - (face == 3);
+        b.face[2] = (face == 4) - (face == 5);
+        std::vector<int> u0, u1;
+        std::vector<unsigned short> u2;
+        if ((rc = setup_select(g, b, kind, 2, src, mask, u0, u1, u2, P.outer[gz][dd]))) return rc;
+      }
+  // CFL timestep: dt = CFL * min over all points of min_i ds_i
+  double ds_min = 1e38;
+  if (d.coord == BHAH_CARTESIAN) {
+    ds_min = fmin(P.dxx[0], fmin(P.dxx[1], P.dxx[2]));
+  } else {
+    const int nb = (int)std::min<long>(VI_MAX_BLOCKS, (P.Ntot + 255) / 256);
+    hipLaunchKernelGGL(setup_cfl_kernel, dim3(nb), dim3(256), 0, nullptr, g, partial);
+    HIP_TRY(hipGetLastError());
+    std::vector<double> h(nb);
+    HIP_TRY(hipMemcpy(h.data(), partial, sizeof(double) * nb, hipMemcpyDeviceToHost));
+    for (double v : h) ds_min = v < ds_min ? v : ds_min;
+  }
+  P.dt = d.CFL_FACTOR * ds_min;
+  return BHAH_OK;
+}
+
+// Upload the set-up to the device and finish the patch (both set-up paths).
+static int patch_upload(bhah_patch *P, const std::vector<double> &rfm0, const std::vector<double> &rfm1,
+                        bhah_patch **out) {
+  const bhah_patch_desc &d = P->d;
+  int rc;
   std::vector<unsigned short> mask(P->inner_dst.size());
   std::vector<int> dst(P->inner_dst.size()), src(P->inner_dst.size());
   for (size_t p = 0; p < mask.size(); p++) {
@@ -925,6 +1383,7 @@ int bhah_hip_patch_create(const bhah_patch_desc *desc, bhah_patch **out) {
   }
   std::vector<int> par(kBssnParity, kBssnParity + 24);
   if ((rc = upload(rfm0, &P->d_rfm0)) || (rc = upload(rfm1, &P->d_rfm1)) || (rc = upload(P->xx[0], &P->d_xx0)) ||
+      (rc = upload(P->xx[1], &P->d_xx1)) || (rc = upload(P->xx[2], &P->d_xx2)) ||
       (rc = upload(dst, &P->d_inner_dst)) || (rc = upload(src, &P->d_inner_src)) ||
       (rc = upload(mask, &P->d_inner_mask)) || (rc = upload(ws, &P->d_wavespeed)) || (rc = upload(fi, &P->d_finf)) ||
       (rc = upload(par, &P->d_gf_parity))) {
@@ -961,7 +1420,7 @@ int bhah_hip_patch_create(const bhah_patch_desc *desc, bhah_patch **out) {
       P->n_rad_lines = (int)(L0.size() / 6);
     }
   }
-  if (hipMalloc((void **)&P->d_red, 8 * sizeof(double)) != hipSuccess) {
+  if (hipMalloc((void **)&P->d_red, (VI_MAX_BLOCKS + 1) * VI_SUMS * sizeof(double)) != hipSuccess) {
     bhah_hip_patch_destroy(P);
     return fail(BHAH_ERR_HIP, "hipMalloc failed");
   }
@@ -976,6 +1435,8 @@ int bhah_hip_patch_destroy(bhah_patch *P) {
   hipFree(P->d_rfm0);
   hipFree(P->d_rfm1);
   hipFree(P->d_xx0);
+  hipFree(P->d_xx1);
+  hipFree(P->d_xx2);
   hipFree(P->d_inner_dst);
   hipFree(P->d_inner_src);
   hipFree(P->d_inner_mask);
@@ -1003,7 +1464,7 @@ int bhah_hip_patch_get_info(const bhah_patch *P, bhah_patch_info *info) {
     info->dxx[i] = P->dxx[i];
     info->invdxx[i] = P->invdxx[i];
   }
-  info->NGHOSTS = P->NG;
+  info->nghosts = P->NG;
   info->num_evol_gfs = P->ngfs;
   info->num_auxevol_gfs = P->naux;
   info->Ntot = P->Ntot;
@@ -1044,6 +1505,13 @@ int bhah_hip_patch_get_inner_bc(const bhah_patch *P, int64_t *dst, int64_t *src,
   memcpy(dst, P->inner_dst.data(), sizeof(int64_t) * P->inner_dst.size());
   memcpy(src, P->inner_src.data(), sizeof(int64_t) * P->inner_src.size());
   memcpy(par, P->inner_parity.data(), P->inner_parity.size());
+  return BHAH_OK;
+}
+
+int bhah_hip_patch_get_rfm(const bhah_patch *P, double *rfm0, double *rfm1) {
+  if (!P || !rfm0 || !rfm1) return fail(BHAH_ERR_ARG, "NULL argument");
+  HIP_TRY(hipMemcpy(rfm0, P->d_rfm0, sizeof(double) * 4 * P->Nx[0], hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(rfm1, P->d_rfm1, sizeof(double) * 3 * P->Nx[1], hipMemcpyDeviceToHost));
   return BHAH_OK;
 }
 
@@ -1167,12 +1635,13 @@ int bhah_hip_MoL_malloc_intermediate_stage_gfs(bhah_patch *P, bhah_MoL_gridfunct
   HIP_TRY(hipMalloc((void **)&gf->y_nplus1_running_total_gfs, bytes));
   HIP_TRY(hipMalloc((void **)&gf->k_odd_gfs, bytes));
   HIP_TRY(hipMalloc((void **)&gf->k_even_gfs, bytes));
-  if (P->naux) HIP_TRY(hipMalloc((void **)&gf->auxevol_gfs, sizeof(double) * (size_t)P->Ntot * P->naux));
+  const bool new_aux = P->naux && !gf->auxevol_gfs;
+  if (new_aux) HIP_TRY(hipMalloc((void **)&gf->auxevol_gfs, sizeof(double) * (size_t)P->Ntot * P->naux));
   const size_t all = bytes;
   HIP_TRY(hipMemsetAsync(gf->y_nplus1_running_total_gfs, 0, all, P->stream));
   HIP_TRY(hipMemsetAsync(gf->k_odd_gfs, 0, all, P->stream));
   HIP_TRY(hipMemsetAsync(gf->k_even_gfs, 0, all, P->stream));
-  if (P->naux) HIP_TRY(hipMemsetAsync(gf->auxevol_gfs, 0, sizeof(double) * (size_t)P->Ntot * P->naux, P->stream));
+  if (new_aux) HIP_TRY(hipMemsetAsync(gf->auxevol_gfs, 0, sizeof(double) * (size_t)P->Ntot * P->naux, P->stream));
   return BHAH_OK;
 }
 
@@ -1227,25 +1696,36 @@ static int enqueue_step(bhah_patch *P, bhah_MoL_gridfunctions *gf, double dt) {
   return BHAH_OK;
 }
 
-int bhah_hip_MoL_step_forward_in_time(bhah_patch *P, bhah_MoL_gridfunctions *gf, double dt, double *time) {
-  if (!P || !gf || !gf->y_n_gfs || !gf->k_odd_gfs || !gf->k_even_gfs || !gf->y_nplus1_running_total_gfs ||
+// The reference's time bookkeeping after a step: time from the iteration count, so that
+// rounding does not accumulate (MoL_step_forward_in_time.py:473-481).
+static void advance_clock(bhah_clock *c) {
+  c->time = c->t_0 + (double)(c->nn - c->nn_0 + 1) * c->dt;
+  c->nn++;
+}
+
+int bhah_hip_MoL_step_forward_in_time(bhah_patch *P, bhah_MoL_gridfunctions *gf, bhah_clock *clk) {
+  if (!P || !gf || !clk || !gf->y_n_gfs || !gf->k_odd_gfs || !gf->k_even_gfs || !gf->y_nplus1_running_total_gfs ||
       (P->naux && !gf->auxevol_gfs))
-    return fail(BHAH_ERR_ARG, "MoL gridfunctions not allocated");
-  int rc = enqueue_step(P, gf, dt);
+    return fail(BHAH_ERR_ARG, "MoL gridfunctions or clock not set");
+  int rc = enqueue_step(P, gf, clk->dt);
   if (rc) return rc;
-  if (time) *time += dt;
+  advance_clock(clk);
   return BHAH_OK;
 }
 
-int bhah_hip_MoL_step_n(bhah_patch *P, bhah_MoL_gridfunctions *gf, double dt, int nsteps, int use_graph) {
-  if (!P || !gf || nsteps < 0) return fail(BHAH_ERR_ARG, "bad arguments");
+int bhah_hip_MoL_step_n(bhah_patch *P, bhah_MoL_gridfunctions *gf, bhah_clock *clk, int nsteps, int use_graph) {
+  if (!P || !gf || !clk || nsteps < 0) return fail(BHAH_ERR_ARG, "bad arguments");
   if (!use_graph) {
     for (int n = 0; n < nsteps; n++) {
-      int rc = bhah_hip_MoL_step_forward_in_time(P, gf, dt, nullptr);
+      int rc = bhah_hip_MoL_step_forward_in_time(P, gf, clk);
       if (rc) return rc;
     }
     return BHAH_OK;
   }
+  if (!gf->y_n_gfs || !gf->k_odd_gfs || !gf->k_even_gfs || !gf->y_nplus1_running_total_gfs ||
+      (P->naux && !gf->auxevol_gfs))
+    return fail(BHAH_ERR_ARG, "MoL gridfunctions not allocated");
+  const double dt = clk->dt;
   const bool reuse = P->graph_exec && memcmp(&P->graph_gf, gf, sizeof(*gf)) == 0 && P->graph_dt == dt;
   if (!reuse) {
     if (P->graph_exec) {
@@ -1265,16 +1745,22 @@ int bhah_hip_MoL_step_n(bhah_patch *P, bhah_MoL_gridfunctions *gf, double dt, in
     hipGraph_t graph;
     hipError_t e = hipStreamEndCapture(cap, &graph);
     P->stream = saved;
-    if (rc) return rc;
+    if (tmp_stream) hipStreamDestroy(cap);
+    if (rc) {
+      if (e == hipSuccess) hipGraphDestroy(graph);
+      return rc;
+    }
     if (e != hipSuccess) return fail(BHAH_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
     e = hipGraphInstantiate(&P->graph_exec, graph, nullptr, nullptr, 0);
     hipGraphDestroy(graph);
-    if (tmp_stream) hipStreamDestroy(cap);
     if (e != hipSuccess) return fail(BHAH_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
     P->graph_gf = *gf;
     P->graph_dt = dt;
   }
-  for (int n = 0; n < nsteps; n++) HIP_TRY(hipGraphLaunch(P->graph_exec, P->stream));
+  for (int n = 0; n < nsteps; n++) {
+    HIP_TRY(hipGraphLaunch(P->graph_exec, P->stream));
+    advance_clock(clk);
+  }
   return BHAH_OK;
 }
 
@@ -1346,24 +1832,46 @@ int bhah_hip_wave_apply_bcs(bhah_patch *P, double *gfs) {
 }
 
 int bhah_hip_diagnostics_constraints(bhah_patch *P, const double *aux, const double *in, double *diag,
-                                     double *l2_H) {
+                                     bhah_volume_integrals *integrals) {
   if (!P || !P->k) return fail(BHAH_ERR_ARG, "patch is not a BSSN patch");
   if (!aux || !in || !diag) return fail(BHAH_ERR_ARG, "NULL argument");
   dim3 g, b;
   P->interior_dims(g, b);
   HIP_TRY(P->k->constraints(g, b, P->stream, in, aux, diag, P->d_rfm0, P->d_rfm1, P->kp(0, 0)));
-  if (l2_H) {
-    HIP_TRY(hipMemsetAsync(P->d_red, 0, 3 * sizeof(double), P->stream));
-    hipLaunchKernelGGL(l2_reduce_kernel, g, b, 0, P->stream, diag, diag, P->d_rfm0, P->d_rfm1, P->kp(0, 0),
-                       fabs(P->dxx[0] * P->dxx[1] * P->dxx[2]), P->d_red);
-    HIP_TRY(hipGetLastError());
-    double red[3];
-    HIP_TRY(hipMemcpyAsync(red, P->d_red, 3 * sizeof(double), hipMemcpyDeviceToHost, P->stream));
-    HIP_TRY(hipStreamSynchronize(P->stream));
-    l2_H[0] = sqrt(red[1] / red[0]);  // RMS over the proper volume
-    l2_H[1] = sqrt(red[1]);           // sqrt of the volume integral of H^2
-    l2_H[2] = red[0];                 // proper volume
-  }
+  if (!integrals) return BHAH_OK;
+  int rc = enqueue_volume_integrals(P, diag, P->d_red, P->d_red + VI_MAX_BLOCKS * VI_SUMS);
+  if (rc) return rc;
+  double sums[VI_SUMS];
+  HIP_TRY(hipMemcpyAsync(sums, P->d_red + VI_MAX_BLOCKS * VI_SUMS, sizeof(sums), hipMemcpyDeviceToHost, P->stream));
+  HIP_TRY(hipStreamSynchronize(P->stream));
+  finish_volume_integrals(sums, integrals);
+  return BHAH_OK;
+}
+
+int bhah_hip_diagnostics(bhah_patch *P, const bhah_clock *clk, double every, const double *y_n, double *aux,
+                         double *diag, bhah_volume_integrals *integrals, int *output_step) {
+  if (!P || !P->k) return fail(BHAH_ERR_ARG, "patch is not a BSSN patch");
+  if (!clk || !y_n || !aux || !diag || !integrals || !output_step) return fail(BHAH_ERR_ARG, "NULL argument");
+  if (!(every > 0.0)) return fail(BHAH_ERR_ARG, "diagnostics_output_every must be > 0 (have %g)", every);
+  *output_step = 0;
+  // output step rule of the reference driver (diagnostics.py:195)
+  if (!(fabs(round(clk->time / every) * every - clk->time) < 0.5 * clk->dt)) return BHAH_OK;
+  int rc;
+  if ((rc = bhah_hip_Ricci_eval(P, y_n, aux))) return rc;
+  if ((rc = bhah_hip_diagnostics_constraints(P, aux, y_n, diag, integrals))) return rc;
+  *output_step = 1;
+  return BHAH_OK;
+}
+
+int bhah_hip_diagnostic_gfs_malloc(const bhah_patch *P, double **diag) {
+  if (!P || !diag) return fail(BHAH_ERR_ARG, "NULL argument");
+  *diag = nullptr;
+  HIP_TRY(hipMalloc((void **)diag, sizeof(double) * BHAH_NUM_DIAG_GFS * (size_t)P->Ntot));
+  return BHAH_OK;
+}
+
+int bhah_hip_diagnostic_gfs_free(double *diag) {
+  HIP_TRY(hipFree(diag));
   return BHAH_OK;
 }
 
@@ -1413,7 +1921,18 @@ extern "C" int bhah_hip_checkpoint_unpack(bhah_patch *P, const void *payload, in
     if (ind[i] < 0 || ind[i] >= P->Ntot) return fail(BHAH_ERR_ARG, "invalid gridpoint index %d at element %d", ind[i], i);
     if (i > 0 && ind[i] <= ind[i - 1]) return fail(BHAH_ERR_ARG, "non-increasing gridpoint index at element %d", i);
   }
-  if (!count) return BHAH_OK;
+  // the reference reader zeroes every point of y_n, then scatters the listed ones
+  // (read_checkpoint__openmp.c:163-171)
+  {
+    dim3 g, b;
+    launch_1d(P->Ntot * P->ngfs, g, b);
+    hipLaunchKernelGGL(fill_kernel, g, b, 0, P->stream, y_n, 0.0, P->Ntot * P->ngfs);
+    HIP_TRY(hipGetLastError());
+  }
+  if (!count) {
+    HIP_TRY(hipStreamSynchronize(P->stream));
+    return BHAH_OK;
+  }
   const long nv = (long)count * P->ngfs;
   int *dind = nullptr;
   double *ddat = nullptr;

@@ -51,6 +51,7 @@ class PatchDesc(ctypes.Structure):
         ("KO_strength_gauge", ctypes.c_double),
         ("KO_strength_nongauge", ctypes.c_double),
         ("wavespeed", ctypes.c_double),
+        ("setup_on_device", ctypes.c_int),
     ]
 
 
@@ -64,10 +65,46 @@ class MoLGridfunctions(ctypes.Structure):
     ]
 
 
+class Clock(ctypes.Structure):
+    """bhah_clock: the time members of commondata_struct (time, t_0, dt, nn, nn_0)."""
+
+    _fields_ = [
+        ("time", ctypes.c_double),
+        ("t_0", ctypes.c_double),
+        ("dt", ctypes.c_double),
+        ("nn", ctypes.c_int),
+        ("nn_0", ctypes.c_int),
+    ]
+
+
+NUM_DIAG_GFS = 6  # H, MU0, MU1, MU2, MSQUARED, LAMBDA_CONSTRAINT
+DIAG_RECIPES = ("whole_domain", "sphere_R_gt_8")
+DIAG_INTEGRANDS = ("H", "MSQUARED", "LAMBDA_CONSTRAINT")
+
+
+class VolumeIntegrals(ctypes.Structure):
+    _fields_ = [
+        ("volume", ctypes.c_double * 2),
+        ("integral", (ctypes.c_double * 3) * 2),
+        ("L2", (ctypes.c_double * 3) * 2),
+        ("RMS", (ctypes.c_double * 3) * 2),
+    ]
+
+    def as_dict(self) -> dict:
+        """{recipe: {"volume": V, f: {"integral": I, "L2": L, "RMS": R}}}"""
+        out = {}
+        for q, name in enumerate(DIAG_RECIPES):
+            d = {"volume": self.volume[q]}
+            for k, f in enumerate(DIAG_INTEGRANDS):
+                d[f] = {"integral": self.integral[q][k], "L2": self.L2[q][k], "RMS": self.RMS[q][k]}
+            out[name] = d
+        return out
+
+
 class PatchInfo(ctypes.Structure):
     _fields_ = [
         ("Nxx", ctypes.c_int * 3),
-        ("NGHOSTS", ctypes.c_int),
+        ("nghosts", ctypes.c_int),
         ("Nxx_plus_2NGHOSTS", ctypes.c_int * 3),
         ("num_evol_gfs", ctypes.c_int),
         ("num_auxevol_gfs", ctypes.c_int),
@@ -98,6 +135,7 @@ SIGNATURES: Dict[str, tuple] = {
     "bhah_hip_patch_get_inner_bc": (_I, [_P, _P, _P, _P]),
     "bhah_hip_patch_get_outer_bc_count": (_I, [_P, _I, _I]),
     "bhah_hip_patch_get_outer_bc": (_I, [_P, _I, _I, _P]),
+    "bhah_hip_patch_get_rfm": (_I, [_P, _P, _P]),
     "bhah_hip_rhs_eval": (_I, [_P, _P, _P, _P]),
     "bhah_hip_Ricci_eval": (_I, [_P, _P, _P]),
     "bhah_hip_enforce_detgammabar_equals_detgammahat": (_I, [_P, _P]),
@@ -106,8 +144,8 @@ SIGNATURES: Dict[str, tuple] = {
     "bhah_hip_apply_bcs_inner_only": (_I, [_P, _P]),
     "bhah_hip_MoL_malloc_intermediate_stage_gfs": (_I, [_P, ctypes.POINTER(MoLGridfunctions)]),
     "bhah_hip_MoL_free": (_I, [_P, ctypes.POINTER(MoLGridfunctions)]),
-    "bhah_hip_MoL_step_forward_in_time": (_I, [_P, ctypes.POINTER(MoLGridfunctions), _D, ctypes.POINTER(_D)]),
-    "bhah_hip_MoL_step_n": (_I, [_P, ctypes.POINTER(MoLGridfunctions), _D, _I, _I]),
+    "bhah_hip_MoL_step_forward_in_time": (_I, [_P, ctypes.POINTER(MoLGridfunctions), ctypes.POINTER(Clock)]),
+    "bhah_hip_MoL_step_n": (_I, [_P, ctypes.POINTER(MoLGridfunctions), ctypes.POINTER(Clock), _I, _I]),
     "bhah_hip_patch_num_schedules": (_I, [_P, _I]),
     "bhah_hip_patch_schedule_name": (ctypes.c_char_p, [_P, _I, _I]),
     "bhah_hip_patch_select_schedule": (_I, [_P, _I, _I]),
@@ -115,7 +153,11 @@ SIGNATURES: Dict[str, tuple] = {
     "bhah_hip_patch_set_tile": (_I, [_P, _I, _I]),
     "bhah_hip_wave_rhs_eval": (_I, [_P, _P, _P]),
     "bhah_hip_wave_apply_bcs": (_I, [_P, _P]),
-    "bhah_hip_diagnostics_constraints": (_I, [_P, _P, _P, _P, ctypes.POINTER(_D)]),
+    "bhah_hip_diagnostics_constraints": (_I, [_P, _P, _P, _P, ctypes.POINTER(VolumeIntegrals)]),
+    "bhah_hip_diagnostics": (_I, [_P, ctypes.POINTER(Clock), _D, _P, _P, _P, ctypes.POINTER(VolumeIntegrals),
+                                  ctypes.POINTER(_I)]),
+    "bhah_hip_diagnostic_gfs_malloc": (_I, [_P, ctypes.POINTER(_P)]),
+    "bhah_hip_diagnostic_gfs_free": (_I, [_P]),
     "bhah_hip_MoL_step_adjoint": (_I, [_P, _P, _D, _P]),
     "bhah_hip_checkpoint_payload_bytes": (ctypes.c_int64, [_P]),
     "bhah_hip_checkpoint_pack": (_I, [_P, _P, _P]),
@@ -181,6 +223,7 @@ class PatchConfig:
     KO_strength_gauge: float = 0.3
     KO_strength_nongauge: float = 0.3
     wavespeed: float = 1.0
+    setup_on_device: bool = False  # reference-metric arrays, boundary structs and CFL on the device
 
     def to_c(self) -> PatchDesc:
         d = PatchDesc()
@@ -199,6 +242,7 @@ class PatchConfig:
         d.KO_strength_gauge = self.KO_strength_gauge
         d.KO_strength_nongauge = self.KO_strength_nongauge
         d.wavespeed = self.wavespeed
+        d.setup_on_device = int(bool(self.setup_on_device))
         return d
 
 
@@ -230,7 +274,7 @@ class Patch:
         info = PatchInfo()
         _check(self.lib.bhah_hip_patch_get_info(self.handle, ctypes.byref(info)))
         self.info = info
-        self.NG = info.NGHOSTS
+        self.NG = info.nghosts
         self.Nx = tuple(info.Nxx_plus_2NGHOSTS)
         self.Ntot = int(info.Ntot)
         self.ngfs = info.num_evol_gfs
@@ -337,10 +381,38 @@ class Patch:
     def wave_apply_bcs(self, gfs):
         _check(self.lib.bhah_hip_wave_apply_bcs(self.handle, _ptr(gfs)))
 
-    def diagnostics_constraints(self, auxevol, in_gfs, diag):
-        out = (ctypes.c_double * 3)()
-        _check(self.lib.bhah_hip_diagnostics_constraints(self.handle, _ptr(auxevol), _ptr(in_gfs), _ptr(diag), out))
-        return out[0], out[1], out[2]
+    def _check_diag(self, diag):
+        if diag.numel() < NUM_DIAG_GFS * self.Ntot:
+            raise BHaHError(f"diag must hold {NUM_DIAG_GFS} gridfunctions ({NUM_DIAG_GFS * self.Ntot} values), "
+                            f"has {diag.numel()}")
+
+    def rfm(self):
+        """Host copies of the reference-metric precompute arrays: rfm0 (4, Nx0) = f0 = r(xx0)
+        and its first three derivatives, rfm1 (3, Nx1) = sin(xx1) and its first two."""
+        import numpy as np
+
+        r0, r1 = np.zeros((4, self.Nx[0])), np.zeros((3, self.Nx[1]))
+        _check(self.lib.bhah_hip_patch_get_rfm(self.handle, r0.ctypes.data, r1.ctypes.data))
+        return r0, r1
+
+    def diagnostics_constraints(self, auxevol, in_gfs, diag, integrals: bool = True):
+        """Constraints H, MU^i, M^2, |Lambda constraint| into diag (6 gridfunctions); with
+        ``integrals`` returns the two recipes' volume integrals as a dict."""
+        self._check_diag(diag)
+        out = VolumeIntegrals()
+        _check(self.lib.bhah_hip_diagnostics_constraints(self.handle, _ptr(auxevol), _ptr(in_gfs), _ptr(diag),
+                                                         ctypes.byref(out) if integrals else None))
+        return out.as_dict() if integrals else None
+
+    def diagnostics(self, clock: "Clock", every: float, y_n, auxevol, diag):
+        """The reference's diagnostics() for this grid: None on non-output steps, else the
+        volume integrals (Ricci and constraint gridfunctions are left in auxevol and diag)."""
+        self._check_diag(diag)
+        out = VolumeIntegrals()
+        flag = ctypes.c_int(0)
+        _check(self.lib.bhah_hip_diagnostics(self.handle, ctypes.byref(clock), every, _ptr(y_n), _ptr(auxevol),
+                                             _ptr(diag), ctypes.byref(out), ctypes.byref(flag)))
+        return out.as_dict() if flag.value else None
 
     def synchronize(self):
         _check(self.lib.bhah_hip_synchronize(self.handle))
@@ -382,22 +454,37 @@ class MoLState:
         self.c = MoLGridfunctions(
             _ptr(self.y_n), _ptr(self.y_run), _ptr(self.k_odd), _ptr(self.k_even), _ptr(self.aux)
         )
-        self.time = 0.0
-        self.nn = 0
+        self.clock = Clock(0.0, 0.0, patch.dt, 0, 0)
+
+    @property
+    def time(self) -> float:
+        return self.clock.time
+
+    @property
+    def nn(self) -> int:
+        return self.clock.nn
+
+    def reset_clock(self, time: float = 0.0, nn: int = 0) -> None:
+        """Set time and nn and restart the time formula there (t_0 = time, nn_0 = nn), as the
+        reference does after reading a checkpoint."""
+        self.clock.time, self.clock.t_0, self.clock.nn, self.clock.nn_0 = time, time, nn, nn
+
+    def _set_dt(self, dt: Optional[float]) -> None:
+        dt = self.patch.dt if dt is None else dt
+        if dt != self.clock.dt:
+            # a new dt restarts the time formula at the current time (as the reference's regrid)
+            self.clock.t_0, self.clock.nn_0, self.clock.dt = self.clock.time, self.clock.nn, dt
 
     def step(self, dt: Optional[float] = None):
-        """MoL_step_forward_in_time: one RK4 step."""
-        dt = self.patch.dt if dt is None else dt
-        t = ctypes.c_double(self.time)
-        _check(self.patch.lib.bhah_hip_MoL_step_forward_in_time(self.patch.handle, ctypes.byref(self.c), dt, ctypes.byref(t)))
-        self.nn += 1
-        self.time = self.nn * dt
+        """MoL_step_forward_in_time: one RK4 step, then time = t_0 + (nn - nn_0 + 1) dt, nn++."""
+        self._set_dt(dt)
+        _check(self.patch.lib.bhah_hip_MoL_step_forward_in_time(self.patch.handle, ctypes.byref(self.c),
+                                                                ctypes.byref(self.clock)))
 
     def steps(self, n: int, dt: Optional[float] = None, use_graph: bool = True):
-        dt = self.patch.dt if dt is None else dt
-        _check(self.patch.lib.bhah_hip_MoL_step_n(self.patch.handle, ctypes.byref(self.c), dt, n, int(use_graph)))
-        self.nn += n
-        self.time = self.nn * dt
+        self._set_dt(dt)
+        _check(self.patch.lib.bhah_hip_MoL_step_n(self.patch.handle, ctypes.byref(self.c), ctypes.byref(self.clock), n,
+                                                  int(use_graph)))
 
 
 class Ensemble:
@@ -452,8 +539,10 @@ class Ensemble:
     def steps(self, n: int, use_graph: bool = True):
         _check(self.lib.bhah_hip_ensemble_step_n(self.handle, n, int(use_graph)))
         for st, dt in zip(self.states, self.dts):
-            st.nn += n
-            st.time = st.nn * dt
+            st._set_dt(dt)
+            for _ in range(n):
+                st.clock.time = st.clock.t_0 + (st.clock.nn - st.clock.nn_0 + 1) * dt
+                st.clock.nn += 1
 
     def synchronize(self):
         _check(self.lib.bhah_hip_ensemble_synchronize(self.handle))

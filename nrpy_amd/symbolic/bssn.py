@@ -500,9 +500,10 @@ def detgamma_enforce_exprs(coord: str) -> Dict[str, sp.Expr]:
 
 
 def constraint_exprs(coord: str, ricci_symbolic: bool = True) -> Dict[str, sp.Expr]:
-    """Hamiltonian constraint H, momentum constraint MU^i and M^2.
+    """Hamiltonian constraint H, momentum constraint MU^i, M^2 and the magnitude of the
+    conformal-connection constraint |Lambdabar^i - Delta^i|.
 
-    Follows ``BSSN_constraints.py:82-183``; with ``ricci_symbolic`` RbarDD is read
+    Follows ``BSSN_constraints.py:82-203``; with ``ricci_symbolic`` RbarDD is read
     from the auxiliary Ricci gridfunctions, otherwise expanded from hDD.
     """
     rfm = reference_metric(coord)
@@ -531,4 +532,7 @@ def constraint_exprs(coord: str, ricci_symbolic: bool = True) -> Dict[str, sp.Ex
     MU = [m * Bq.exp_m4phi for m in MU]
     # M^2 = gammabar_ij MU^i MU^j / e^{-4 phi} (BSSN_constraints.py:177-182)
     Msq = sum(Bq.gammabarDD[i][j] * MU[i] * MU[j] for i in I3 for j in I3) / Bq.exp_m4phi
-    return {"H": H, "MU0": MU[0], "MU1": MU[1], "MU2": MU[2], "MSQUARED": Msq}
+    # |Lambdabar^i - Delta^i| in the conformal metric (BSSN_constraints.py:190-203)
+    LamU = [Bq.LambdabarU[i] - Bq.DGammaU[i] for i in I3]
+    LamSq = sum(Bq.gammabarDD[i][j] * LamU[i] * LamU[j] for i in I3 for j in I3)
+    return {"H": H, "MU0": MU[0], "MU1": MU[1], "MU2": MU[2], "MSQUARED": Msq, "LAMBDA": sp.sqrt(LamSq)}

@@ -695,9 +695,12 @@ int og_checkpoint_unpack(long ntot, int ngf, const void *in, double *y) {
   if (count < 0 || count > ntot) return -1;
   const int *ind = (const int *)(c + sizeof(int));
   const double *compact = (const double *)(c + sizeof(int) * (1 + (long)count));
-  for (int i = 0; i < count; i++) {
+  /* validate every index first (read_checkpoint__openmp.c:130-145) */
+  for (int i = 0; i < count; i++)
     if (ind[i] < 0 || ind[i] >= ntot || (i > 0 && ind[i] <= ind[i - 1])) return -1;
+  /* zero all of y_n, then scatter the listed points (read_checkpoint__openmp.c:163-171) */
+  for (long i = 0; i < ntot * ngf; i++) y[i] = 0.0;
+  for (int i = 0; i < count; i++)
     for (int gf = 0; gf < ngf; gf++) y[ntot * gf + ind[i]] = compact[(long)i * ngf + gf];
-  }
   return 0;
 }

@@ -142,9 +142,41 @@ class OracleGrid:
         return out
 
     def constraints(self, in_gfs, aux) -> np.ndarray:
-        out = self.gfs(5)
+        """H, MU0, MU1, MU2, MSQUARED, LAMBDA_CONSTRAINT at interior points (6 gridfunctions)."""
+        out = self.gfs(6)
         lib().og_constraints(ctypes.byref(self.g), self.variant, _p(self.rfm0), _p(self.rfm1), _p(aux),
                              _p(in_gfs), _p(out))
+        return out
+
+    def volume_integrals(self, diag) -> dict:
+        """The two diagnostics recipes over interior points, restated in numpy
+        (diagnostics_volume_integration.py:177-207; diagnostics_volume_integration_helpers.h:396-476):
+        "whole_domain" and "sphere_R_gt_8" (|x| > 8), integrands H^2, MSQUARED^2 and
+        LAMBDA_CONSTRAINT^2 against dV = sqrt(det gammahat) |dxx0 dxx1 dxx2|, with L2 = sqrt(I)
+        and RMS = sqrt(I / V) (0 for an empty volume, helpers.h:332-345)."""
+        NG = self.NG
+        sl = (slice(NG, -NG),) * 3
+        f0 = self.rfm0[: self.Nx[0]][NG:-NG]
+        f0p = self.rfm0[self.Nx[0]: 2 * self.Nx[0]][NG:-NG]
+        s1 = self.rfm1[: self.Nx[1]][NG:-NG]
+        dxx = abs(self.g.dxx[0] * self.g.dxx[1] * self.g.dxx[2])
+        dV = np.abs(f0p[None, None, :] * f0[None, None, :] ** 2 * s1[None, :, None]) * dxx
+        dV = np.broadcast_to(dV, (self.Nx[2] - 2 * NG, self.Nx[1] - 2 * NG, self.Nx[0] - 2 * NG))
+        th = self.xx[1][NG:-NG][None, :, None]
+        ph = self.xx[2][NG:-NG][:, None, None]
+        r = f0[None, None, :]
+        X, Y, Z = r * np.sin(th) * np.cos(ph), r * np.sin(th) * np.sin(ph), r * np.cos(th) + 0 * ph
+        outside = ~(np.sqrt(X * X + Y * Y + Z * Z) <= 8.0)
+        fields = {"H": diag[0][sl], "MSQUARED": diag[4][sl], "LAMBDA_CONSTRAINT": diag[5][sl]}
+        out = {}
+        for name, keep in (("whole_domain", np.ones_like(outside)), ("sphere_R_gt_8", outside)):
+            w = np.where(keep & (dV != 0), dV, 0.0)
+            vol = float(w.sum())
+            d = {"volume": vol}
+            for f, v in fields.items():
+                I = float((v * v * w).sum())
+                d[f] = {"integral": I, "L2": np.sqrt(max(I, 0.0)), "RMS": np.sqrt(max(I, 0.0) / vol) if vol > 0 else 0.0}
+            out[name] = d
         return out
 
     def wave_rhs(self, in_gfs, wavespeed=1.0) -> np.ndarray:

@@ -84,6 +84,7 @@ struct bhah_cpu_barrier_t {
 inline bhah_cpu_barrier_t bhah_cpu_bar;
 #define __syncthreads() bhah_cpu_bar.wait()
 #define BHAH_BARRIER() bhah_cpu_bar.wait()
+#define BHAH_BARRIER_PHASE(n) bhah_cpu_bar.wait()
 #define BHAH_VMWAIT(n) ((void)0)
 // every lane of a wave holds the same value where the kernels use it (the wave index)
 #define __builtin_amdgcn_readfirstlane(x) (x)

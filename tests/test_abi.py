@@ -36,7 +36,7 @@ def test_python_binding_covers_the_header():
 
 def test_abi_version_call_needs_no_device():
     lib = runtime.load_library()
-    assert lib.bhah_hip_abi_version() == 1
+    assert lib.bhah_hip_abi_version() == 2
 
 
 def test_null_arguments_are_rejected_without_touching_a_device():

@@ -28,3 +28,15 @@ def test_reader_rejects_bad_indices(oracle_lib):
     assert L.og_checkpoint_unpack(2, 1, bad.ctypes.data, y.ctypes.data) == -1
     bad = np.frombuffer(struct.pack("<2i", 5, 0), np.uint8).copy()
     assert L.og_checkpoint_unpack(2, 1, bad.ctypes.data, y.ctypes.data) == -1
+
+
+def test_partial_payload_zeroes_unlisted_points(oracle_lib):
+    """A payload listing only some points: the reader zeroes all of y_n first, so unlisted
+    points come back as 0 whatever y_n held (read_checkpoint__openmp.c:163-171)."""
+    L = oracle_lib.lib()
+    ntot, ngf = 5, 2
+    payload = struct.pack("<3i", 2, 1, 3) + struct.pack("<4d", 1.5, 2.5, 3.5, 4.5)
+    buf = np.frombuffer(payload, np.uint8).copy()
+    y = np.full(ntot * ngf, 7.0)
+    assert L.og_checkpoint_unpack(ntot, ngf, buf.ctypes.data, y.ctypes.data) == 0
+    assert y.tolist() == [0.0, 1.5, 0.0, 3.5, 0.0, 0.0, 2.5, 0.0, 4.5, 0.0]

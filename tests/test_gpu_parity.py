@@ -18,6 +18,8 @@ CASES = {
     "spherical_fd4": dict(coord=1, Nxx=(24, 8, 2), fd_order=4, size=7.5, sinhw=0.2, sym=4, KO=False, variant=0),
     # SinhSpherical, full 3D, 6th order + KO: the 192x96x96 workload's kernels
     "sinhspherical_fd6": dict(coord=2, Nxx=(20, 10, 12), fd_order=6, size=64.0, sinhw=0.2, sym=0, KO=True, variant=1),
+    # SinhSpherical, full 3D, 4th order + KO: the third kernel build
+    "sinhspherical_fd4": dict(coord=2, Nxx=(18, 8, 10), fd_order=4, size=20.0, sinhw=0.2, sym=0, KO=True, variant=1),
 }
 
 
@@ -171,8 +173,8 @@ def test_head_on_collision_over_many_steps(hip, oracle_lib):
     c_o = O.constraints(st_o["y_n"], aux_o)
     aux_d = P.new_gfs(6)
     P.Ricci_eval(S.y_n, aux_d)
-    diag = P.new_gfs(5)
-    P.diagnostics_constraints(aux_d, S.y_n, diag)
+    diag = P.new_gfs(6)
+    P.diagnostics_constraints(aux_d, S.y_n, diag, integrals=False)
     er = rel_err(interior(diag.cpu().numpy(), NG)[:1], interior(c_o, NG)[:1])
     assert er.max() < 1e-9, er
 
@@ -236,20 +238,69 @@ def test_pair_load_schedules_guard_odd_rows_and_unaligned_input(hip, oracle_lib)
     assert rel_err(interior(r.cpu().numpy(), NG), interior(rhs_o, NG)).max() < 1e-12
 
 
+def integrals_close(a, b, rtol):
+    """Two volume-integral dicts agree to rtol (relative to each quantity's magnitude)."""
+    for recipe in b:
+        assert abs(a[recipe]["volume"] - b[recipe]["volume"]) <= rtol * abs(b[recipe]["volume"]), recipe
+        for f, v in b[recipe].items():
+            if f == "volume":
+                continue
+            for key in ("integral", "L2", "RMS"):
+                assert abs(a[recipe][f][key] - v[key]) <= rtol * abs(v[key]) + 1e-300, (recipe, f, key, a, b)
+
+
 @pytest.mark.parametrize("case", list(CASES))
 def test_constraints_match_oracle(hip, oracle_lib, case):
+    """Per-point H, MU^i, M^2 and |Lambda constraint| against the oracle at 1e-11, and the
+    two recipes' volume integrals against a numpy reduction of the oracle's fields (1e-12);
+    repeated calls give bitwise equal integrals (fixed reduction order)."""
     import torch
 
     P, O = make_pair(hip, oracle_lib, case)
     y = state(O, 5)
     aux = O.ricci(y)
     c_o = O.constraints(y, aux)
-    diag = P.new_gfs(5)
-    rms, l2, vol = P.diagnostics_constraints(torch.from_numpy(aux).cuda(), torch.from_numpy(y).cuda(), diag)
+    diag = P.new_gfs(6)
+    auxd, yd = torch.from_numpy(aux).cuda(), torch.from_numpy(y).cuda()
+    vi = P.diagnostics_constraints(auxd, yd, diag)
     NG = O.NG
     er = rel_err(interior(diag.cpu().numpy(), NG), interior(c_o, NG))
     assert er.max() < 1e-11, er
-    assert np.isfinite(rms) and rms > 0 and vol > 0
+    assert np.abs(interior(c_o, NG)[5]).max() > 0  # the perturbed data violate the Lambda constraint
+    integrals_close(vi, O.volume_integrals(c_o), 1e-12)
+    assert P.diagnostics_constraints(auxd, yd, diag) == vi
+    if case == "spherical_fd4":
+        # midpoint rule of the whole domain's proper volume, in closed form:
+        # sum_i r_i^2 dr = dr^3 (N^3/3 - N/12), sum_j sin(theta_j) dtheta = dtheta / sin(dtheta/2)
+        N0, N1 = CASES[case]["Nxx"][:2]
+        dr, dth = CASES[case]["size"] / N0, np.pi / N1
+        vol = dr**3 * (N0**3 / 3 - N0 / 12) * dth / np.sin(dth / 2) * 2 * np.pi
+        assert abs(vi["whole_domain"]["volume"] - vol) < 1e-13 * vol
+        assert vi["sphere_R_gt_8"]["volume"] == 0.0  # r_max = 7.5
+        assert vi["sphere_R_gt_8"]["H"]["RMS"] == 0.0
+
+
+def test_diagnostics_driver_output_steps(hip, oracle_lib):
+    """bhah_hip_diagnostics follows the reference driver's output-step rule and, on an
+    output step, leaves Ricci and the constraints of y_n in auxevol and diag."""
+    import torch
+    from nrpy_amd.runtime import Clock
+
+    P, O = make_pair(hip, oracle_lib, "sinhspherical_fd6")
+    y = state(O, 5)
+    yd = torch.from_numpy(y).cuda()
+    aux, diag = P.new_gfs(6), P.new_gfs(6)
+    dt = P.dt
+    every = 4.0 * dt
+    assert P.diagnostics(Clock(1.3 * dt, 0.0, dt, 1, 0), every, yd, aux, diag) is None
+    assert float(diag.abs().max()) == 0.0
+    vi = P.diagnostics(Clock(4.2 * dt, 0.0, dt, 4, 0), every, yd, aux, diag)
+    assert vi is not None
+    c_o = O.constraints(y, O.ricci(y))
+    NG = O.NG
+    assert rel_err(interior(diag.cpu().numpy(), NG), interior(c_o, NG)).max() < 1e-11
+    integrals_close(vi, O.volume_integrals(c_o), 1e-11)
+    assert vi["sphere_R_gt_8"]["volume"] > 0  # AMPL 64 reaches beyond r = 8
 
 
 def test_wave_matches_oracle_and_exact(hip, oracle_lib):

@@ -118,9 +118,9 @@ ENS_DRIVER = os.path.join(HERE, "kernel_cpu", "ens_driver.cpp")
 
 
 @pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
-@pytest.mark.parametrize("kind,cand,tile", [("ricci", "r2_lb1_sl_pf16", (64, 4, 1)),
-                                            ("rhs", "p3_lb1_sl_ln_pf16", (64, 4, 1)),
-                                            ("ricci", "r1_lb1_sj_L16x8x2x1_d2_dma", (16, 8, 2)),
+@pytest.mark.parametrize("kind,cand,tile", [("ricci", "r1_lb1_pf16_u32", (64, 4, 1)),
+                                            ("rhs", "p1_lb1_sl_lf_pf16_u32_w2", (64, 4, 1)),
+                                            ("ricci", "r1_lb1_sj_L16x8x2x1_d3_dma_rb1", (16, 8, 2)),
                                             ("rhs", "p1_lb1_sj_L16x8x2x1_d3_dma_rb1", (16, 8, 2)),
                                             ("rhs", "p1_lb1_sj_L16x8x2x1_d3_dma_rb1_pg8", (16, 8, 2)),
                                             ("rhs", "p1_lb1_sj_L16x8x2x1_d3_dma_rb1_pg8_ub", (16, 8, 2))])

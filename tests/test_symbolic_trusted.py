@@ -107,7 +107,7 @@ def test_intermediate_quantities():
 
 
 def test_constraints():
-    """H, MU^i, M^2 (BSSN_constraints.py:82-183).
+    """H, MU^i, M^2 and the Lambda constraint (BSSN_constraints.py:82-203).
 
     The fixture was generated with stress-energy sources (BSSN_constraints.py:509,
     "_T4munu"): H += -16 pi rho, MU^i += -8 pi gamma^ij S_j with rho = alpha^2 T^00 and
@@ -133,6 +133,8 @@ def test_constraints():
           ["H", "MU_0", "MU_1", "MU_2", "Msquared"])
     # Lambda-constraint pieces that carry no source terms: Lambdabar^i - Delta^i
     lam = {f"LambdaConstraintU_{i}": Bq.LambdabarU[i] - Bq.DGammaU[i] for i in range(3)}
+    lam["LambdaConstraintMagnitude"] = ex["LAMBDA"]
+    lam["LambdaConstraintSquared"] = ex["LAMBDA"] ** 2
     check(lam, vals, list(lam))
 
 
