@@ -225,6 +225,19 @@ int bhah_hip_wave_apply_bcs(bhah_patch *patch, double *gfs);
 int bhah_hip_diagnostics_constraints(bhah_patch *patch, const double *auxevol_gfs, const double *in_gfs, double *diag,
                                      bhah_volume_integrals *integrals);
 
+/* The same constraint diagnostics and volume integrals, allocating nothing and never
+ * synchronising: no counterpart in the reference.  It is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * reduction's partial sums go to one 64 KiB device array per GPU that the library holds
+ * until the process ends (whatever the grid size, at most 1024 x 8 partial sums), so calls
+ * that overlap on one GPU (other streams, other threads) race and must be serialised by the
+ * caller.  integral_sums (device, 2 * 4 doubles) receives per recipe the proper volume and
+ * the integrals of H^2, MSQUARED^2 and LAMBDA_CONSTRAINT^2 (the volume and integral members of
+ * bhah_volume_integrals, bitwise equal to bhah_hip_diagnostics_constraints'; L2 and RMS are
+ * their square roots).  diag as in bhah_hip_diagnostics_constraints. */
+int bhah_hip_diagnostics_constraints_noalloc(bhah_patch *patch, const double *auxevol_gfs, const double *in_gfs,
+                                             double *diag, double *integral_sums);
+
 /* diagnostics (nrpy/infrastructures/BHaH/diagnostics/diagnostics.py:46-270) for one grid: on
  * output steps, fabs(round(time / every) * every - time) < dt / 2 (diagnostics.py:195), it
  * evaluates Ricci_eval on y_n into auxevol_gfs, the constraint diagnostics into diag and the

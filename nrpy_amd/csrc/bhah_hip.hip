@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -642,6 +643,13 @@ volume_integrals_partial_kernel(const double *__restrict__ diag, const double *_
   if (tid < VI_SUMS) partial[(long)blockIdx.x * VI_SUMS + tid] = sh[tid][0];
 }
 
+// Partial sums of bhah_hip_diagnostics_constraints_noalloc, which allocates nothing: one
+// array per GPU for the life of the process (64 KiB), shared by every call on that GPU.
+// Host code never uses this name as an address; it uses the per-device pointer from
+// hipGetSymbolAddress (noalloc_scratch).
+constexpr int NOALLOC_SCRATCH_DOUBLES = VI_MAX_BLOCKS * VI_SUMS;
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+
 __global__ void __launch_bounds__(64)
 volume_integrals_final_kernel(const double *__restrict__ partial, int nblocks, double *__restrict__ out) {
   const int k = threadIdx.x;
@@ -1029,6 +1037,29 @@ int enqueue_volume_integrals(bhah_patch *P, const double *diag, double *partial,
   return BHAH_OK;
 }
 
+// Device address of `scratch` on the current GPU, looked up once per GPU with that GPU current
+// (each GPU holds its own copy of the array at its own address).
+constexpr int NOALLOC_MAX_DEVICES = 64;
+double *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+std::mutex g_scratch_mu;
+
+int noalloc_scratch(double **out) {
+  *out = nullptr;
+  int dev = -1;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(BHAH_ERR_UNSUPPORTED, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(BHAH_ERR_HIP, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev, hipGetErrorString(e));
+    g_scratch_ptr[dev] = (double *)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return BHAH_OK;
+}
+
 // L2 and RMS columns of diags_integration_write_row (diagnostics_volume_integration_helpers.h:332-345).
 void finish_volume_integrals(const double sums[VI_SUMS], bhah_volume_integrals *r) {
   for (int q = 0; q < BHAH_DIAG_NUM_RECIPES; q++) {
@@ -1123,6 +1154,8 @@ static int patch_create_impl(const bhah_patch_desc *desc, bhah_patch **out, bool
   }
   std::vector<double> rfm0(4 * P->Nx[0]), rfm1(3 * P->Nx[1]);
   int rc = on_device ? setup_on_device(*P, rfm0, rfm1) : setup_on_host(*P, rfm0, rfm1);
+  double *scr = nullptr;
+  if (!rc) rc = noalloc_scratch(&scr);  // set-up of the no-allocation entry point on this GPU
   if (rc) {
     delete P;
     return rc;
@@ -1846,6 +1879,25 @@ int bhah_hip_diagnostics_constraints(bhah_patch *P, const double *aux, const dou
   HIP_TRY(hipStreamSynchronize(P->stream));
   finish_volume_integrals(sums, integrals);
   return BHAH_OK;
+}
+
+int bhah_hip_diagnostics_constraints_noalloc(bhah_patch *P, const double *aux, const double *in, double *diag,
+                                             double *integral_sums) {
+  if (!P || !P->k) return fail(BHAH_ERR_ARG, "patch is not a BSSN patch");
+  if (!aux || !in || !diag || !integral_sums) return fail(BHAH_ERR_ARG, "NULL argument");
+  // the reduction writes at most VI_MAX_BLOCKS x VI_SUMS partial sums whatever the grid size
+  const long nint = (long)P->d.Nxx[0] * P->d.Nxx[1] * P->d.Nxx[2];
+  const long nb = std::min<long>(VI_MAX_BLOCKS, std::max<long>(1, (nint + 255) / 256));
+  if (nb * VI_SUMS > NOALLOC_SCRATCH_DOUBLES)
+    return fail(BHAH_ERR_UNSUPPORTED, "%ld partial sums exceed the %d-value scratch", nb * VI_SUMS,
+                NOALLOC_SCRATCH_DOUBLES);
+  double *scr = nullptr;
+  int rc = noalloc_scratch(&scr);
+  if (rc) return rc;
+  dim3 g, b;
+  P->interior_dims(g, b);
+  HIP_TRY(P->k->constraints(g, b, P->stream, in, aux, diag, P->d_rfm0, P->d_rfm1, P->kp(0, 0)));
+  return enqueue_volume_integrals(P, diag, scr, integral_sums);
 }
 
 int bhah_hip_diagnostics(bhah_patch *P, const bhah_clock *clk, double every, const double *y_n, double *aux,

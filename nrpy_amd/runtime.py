@@ -156,6 +156,7 @@ SIGNATURES: Dict[str, tuple] = {
     "bhah_hip_diagnostics_constraints": (_I, [_P, _P, _P, _P, ctypes.POINTER(VolumeIntegrals)]),
     "bhah_hip_diagnostics": (_I, [_P, ctypes.POINTER(Clock), _D, _P, _P, _P, ctypes.POINTER(VolumeIntegrals),
                                   ctypes.POINTER(_I)]),
+    "bhah_hip_diagnostics_constraints_noalloc": (_I, [_P, _P, _P, _P, _P]),
     "bhah_hip_diagnostic_gfs_malloc": (_I, [_P, ctypes.POINTER(_P)]),
     "bhah_hip_diagnostic_gfs_free": (_I, [_P]),
     "bhah_hip_MoL_step_adjoint": (_I, [_P, _P, _D, _P]),
