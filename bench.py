@@ -176,6 +176,31 @@ def bench_batched(steps: int, warmup: int, n: int = 64) -> dict:
     }
 
 
+def bench_setup() -> dict:
+    """Patch set-up (reference-metric arrays, boundary structs, CFL timestep) on the host loops
+    against the device kernels, for the 192x96x96 grid and for the 64 ensemble members."""
+    import dataclasses
+
+    import torch
+
+    from nrpy_amd.runtime import Patch, PatchConfig
+
+    c = CONFIGS["c3"]
+    c3 = PatchConfig(coord=c["coord"], Nxx=c["Nxx"], fd_order=c["fd_order"], symmetry_axes=c["sym"],
+                     enable_KO=c["KO"], grid_physical_size=c["size"], SINHW=c["sinhw"])
+    c4 = [PatchConfig(Nxx=m["Nxx"], grid_physical_size=10.0, eta=2.0) for m in c4_members()]
+    out = {}
+    for name, cfgs in (("bssn_sinhspherical_192x96x96", [c3]), ("ensemble_64_members", c4)):
+        for where in (False, True):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ps = [Patch(dataclasses.replace(cf, setup_on_device=where), tuned=False) for cf in cfgs]
+            torch.cuda.synchronize()
+            out[f"{name}_{'device' if where else 'host'}_s"] = time.perf_counter() - t0
+            del ps
+    return out
+
+
 def _timed(fn):
     import torch
 
@@ -314,6 +339,7 @@ def main() -> None:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--repeats", type=int, default=5, help="further timed blocks of K steps for the spread")
     ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
@@ -372,6 +398,14 @@ def main() -> None:
     npts = c["Nxx"][0] * c["Nxx"][1] * c["Nxx"][2]
     value = npts * args.steps / sec
     assert torch.isfinite(S.y_n).all().item(), "evolution produced non-finite values"
+    # run-to-run spread: further blocks of the same K steps, reported beside `value`
+    rep_ms = []
+    for _ in range(args.repeats):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        S.steps(args.steps, dt, use_graph=use_graph)
+        torch.cuda.synchronize()
+        rep_ms.append((time.perf_counter() - t1) * 1e3 / args.steps)
 
     # Dominant kernel (rhs_eval) timed on its own stream with HIP events.
     with open(os.path.join(ROOT, "nrpy_amd", "csrc", "gen", "kernel_meta.json")) as f:
@@ -429,8 +463,19 @@ def main() -> None:
             "dt": dt,
             "hipgraph": use_graph,
         },
+        "repeats": {
+            "blocks": len(rep_ms),
+            "steps_each": args.steps,
+            "median_ms_per_step": float(np.median(rep_ms)) if rep_ms else None,
+            "min_ms_per_step": min(rep_ms) if rep_ms else None,
+            "max_ms_per_step": max(rep_ms) if rep_ms else None,
+            "median_value": npts / (float(np.median(rep_ms)) * 1e-3) if rep_ms else None,
+        },
         "roofline": {
+            # compute roofline: the stencil arithmetic runs on the FP64 vector ALUs (no matrix
+            # instructions); MI355X's dense FP64 peak is the same for vector and matrix units
             "bound": "mfma",
+            "compute_unit": "FP64 VALU",
             "compute": "FP64 VALU (stencil arithmetic, no MFMA); peak is MI355X dense FP64",
             "kernel": f"rhs_eval: bssn_rhs_{tag}_fd{c['fd_order']}_{sched}_g*",
             "achieved": achieved,
@@ -448,6 +493,7 @@ def main() -> None:
     if not args.no_small:
         result["wave_c1"] = bench_wave()
         result["bssn_c2"] = bench_c2()
+        result["setup"] = bench_setup()
     if not args.no_batched:
         result["batched"] = bench_batched(args.batched_steps, 2)
     if not args.no_adjoint:

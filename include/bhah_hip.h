@@ -274,6 +274,17 @@ int bhah_hip_checkpoint_unpack(bhah_patch *patch, const void *payload, int64_t b
  * psi = 1 + m1 / (2 |x - x1|) + m2 / (2 |x - x2|).  bh = {m1, x1, y1, z1, m2, x2, y2, z2}. */
 int bhah_hip_initial_data_brill_lindquist(bhah_patch *patch, const double bh[8], double *y_n);
 
+/* General initial data: initial_data_reader__convert_ADM_Cartesian_to_BSSN
+ * (nrpy/infrastructures/BHaH/general_relativity/ADM_Initial_Data_Reader__BSSN_Converter.py:268-580,
+ * 1019-1129) followed by the outer extrapolation initial_data() applies (initial_data.py:252).
+ * adm_cartesian (device, 19 * Ntot doubles) holds ADM data in the Cartesian basis at every
+ * point, ghost zones included: gammaDD00, 01, 02, 11, 12, 22, KDD00, ..., KDD22, alpha,
+ * betaU0-2, BU0-2.  Per point: BSSN in the Cartesian basis (ADM_to_BSSN.py:80-160), the basis
+ * change to the patch's coordinates with the Jacobian of its map, rescaling; lambda^i = 0.
+ * Then inner boundary conditions, lambda^i = Lambdabar^i / ReU^i from finite differences of
+ * hDD at interior points, and apply_bcs_outerextrap_and_inner on y_n. */
+int bhah_hip_initial_data_convert_ADM_Cartesian_to_BSSN(bhah_patch *patch, const double *adm_cartesian, double *y_n);
+
 /* Reverse mode of MoL_step_forward_in_time (one RK4 step; no reference counterpart --
  * the reference has no adjoint).  Given y_n, the state at the start of the step, and
  * lam, the adjoint (gradient of a scalar loss) with respect to the state after the

@@ -127,7 +127,7 @@ class LapseLossProblem:
         for n in range(1, cfg.nsteps + 1):
             if checkpoints is not None and (n - 1) % cfg.checkpoint_every == 0:
                 checkpoints.append(S.y_n.clone())
-            S.step(self.dt)
+            S.steps(1, self.dt, use_graph=True)
             if n >= first:
                 rec.append(self.samples(S.y_n))
         s = torch.stack(rec)
@@ -151,20 +151,25 @@ class LapseLossProblem:
         dLds = 2.0 * (window_samples - self.target_samples) / window_samples.numel()  # [W, n]
         lam = torch.zeros_like(checkpoints[0])
         flat = lam.view(-1)
+        # one state array for the recomputation and one input array for the step adjoint, so
+        # that both run as replays of the graphs captured on their first use
+        R = MoLState(self.P, torch.empty_like(checkpoints[0]))
+        y_in = torch.empty_like(checkpoints[0])
         for c in range(len(checkpoints) - 1, -1, -1):
             n0 = c * K
             n1 = min(n0 + K, cfg.nsteps)
             # recompute y_{n0} .. y_{n1-1} from the checkpoint
-            states = [checkpoints[c].clone()]
-            S = MoLState(self.P, checkpoints[c].clone())
+            states = [checkpoints[c]]
+            R.y_n.copy_(checkpoints[c])
             for _ in range(n0 + 1, n1):
-                S.step(self.dt)
-                states.append(S.y_n.clone())
+                R.steps(1, self.dt, use_graph=True)
+                states.append(R.y_n.clone())
             for n in range(n1, n0, -1):
                 if n >= first:
                     g = (dLds[n - first][:, None] * self.s_w).reshape(-1)
                     flat.index_add_(0, self.s_idx.reshape(-1), g)
-                self.P.MoL_step_adjoint(states[n - 1 - n0], self.dt, lam)
+                y_in.copy_(states[n - 1 - n0])
+                self.P.MoL_step_adjoint(y_in, self.dt, lam)
         _, dy = self._y0(theta)
         lam_np = lam.cpu().numpy()
         return np.array([np.sum(lam_np * dy[k]) for k in range(4)])

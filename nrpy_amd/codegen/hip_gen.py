@@ -1689,6 +1689,21 @@ bssn_constraints_{tag}(const double *__restrict__ in, const double *__restrict__
 {body}
 }}
 """
+    # ---- lambda^i of converted initial data from finite differences of hDD, at interior
+    # points, in place (reads hDD only, writes lambdaU)
+    lspec = specs.initial_lambda_spec(v)
+    lv = [f"gfs[{bssn.EVOL_GFS.index(g)} * Ntot + idx]" for g, _ in lspec.outputs]
+    body = _ordered_body(lspec, fd_order, lv, rats)
+    lam = f"""
+extern "C" __global__ void __launch_bounds__(256)
+bssn_initial_lambda_{tag}(double *gfs, const double *__restrict__ rfm0, const double *__restrict__ rfm1,
+                          const bhah_kparams p) {{
+  const double *in = gfs;
+{_prologue(True)}
+{body}
+}}
+"""
+    cons += lam
     head = [f"// ---- BSSN kernels: {v.coord}, symmetry axes '{v.symmetry_axes}', KO={v.enable_KO}, fd_order={fd_order}"]
     consts = "\n".join(f"#define {k} ({v})" for k, v in sorted(rats.items()))
     undefs = "\n".join(f"#undef {k}" for k in sorted(rats))
@@ -1716,6 +1731,11 @@ extern "C" hipError_t launch_bssn_constraints_{tag}(dim3 g, dim3 b, hipStream_t 
 extern "C" hipError_t launch_bssn_detgamma_{tag}(dim3 g, dim3 b, hipStream_t s, double *gfs, const double *rfm0,
     const double *rfm1, bhah_kparams p) {{
   hipLaunchKernelGGL(bssn_detgamma_{tag}, g, b, 0, s, gfs, rfm0, rfm1, p);
+  return hipGetLastError();
+}}
+extern "C" hipError_t launch_bssn_initial_lambda_{tag}(dim3 g, dim3 b, hipStream_t s, double *gfs, const double *rfm0,
+    const double *rfm1, bhah_kparams p) {{
+  hipLaunchKernelGGL(bssn_initial_lambda_{tag}, g, b, 0, s, gfs, rfm0, rfm1, p);
   return hipGetLastError();
 }}
 """
@@ -1836,6 +1856,7 @@ def generate(outdir: str) -> List[str]:
             f"    {{{COORD_ENUM[v.coord]}, {sym_mask}, {int(v.enable_KO)}, {order}, \"{tag}\", "
             f"{arrays['rhs']}, {ncand['rhs']}, {arrays['ricci']}, {ncand['ricci']}, "
             f"launch_bssn_rk_det_{tag}, launch_bssn_detgamma_{tag}, launch_bssn_constraints_{tag}, "
+            f"launch_bssn_initial_lambda_{tag}, "
             f"{f'launch_bssn_ens_rk_det_{tag}' if ens else 'nullptr'}, "
             + (f"adj_nslots_{tag}, launch_adj_rhs_{tag}, launch_adj_ricci_{tag}, launch_adj_det_{tag}, "
                f"launch_adj_gather_{tag}}},"
@@ -1875,6 +1896,10 @@ def generate(outdir: str) -> List[str]:
         table_decl.append(
             f'extern "C" hipError_t launch_bssn_constraints_{tag}(dim3, dim3, hipStream_t, const double *, '
             f"const double *, double *, const double *, const double *, bhah_kparams);"
+        )
+        table_decl.append(
+            f'extern "C" hipError_t launch_bssn_initial_lambda_{tag}(dim3, dim3, hipStream_t, double *, '
+            f"const double *, const double *, bhah_kparams);"
         )
     for order in WAVE_BUILDS:
         path = os.path.join(outdir, f"wave_fd{order}.hip")

@@ -161,6 +161,16 @@ def detgamma_spec(v: Variant) -> Spec:
 
 
 @functools.lru_cache(maxsize=None)
+def initial_lambda_spec(v: Variant) -> Spec:
+    old = _with_axes(v.symmetry_axes)
+    try:
+        ex = bssn.initial_lambda_exprs(v.coord)
+    finally:
+        tensors.set_symmetry_axes(old)
+    return Spec(f"initial_lambda_{v.tag}", list(ex.items()), list(bssn.EVOL_GFS), [])
+
+
+@functools.lru_cache(maxsize=None)
 def constraints_spec(v: Variant) -> Spec:
     old = _with_axes(v.symmetry_axes)
     try:

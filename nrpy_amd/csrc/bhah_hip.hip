@@ -46,6 +46,7 @@ struct BssnKernels {
   hipError_t (*detgamma)(dim3, dim3, hipStream_t, double *, const double *, const double *, bhah_kparams);
   hipError_t (*constraints)(dim3, dim3, hipStream_t, const double *, const double *, double *, const double *,
                             const double *, bhah_kparams);
+  hipError_t (*initial_lambda)(dim3, dim3, hipStream_t, double *, const double *, const double *, bhah_kparams);
   hipError_t (*ens_rk_det)(dim3, dim3, hipStream_t, const bhah_member *, const bhah_item *, int, int, int, int);
   // reverse-mode kernels of one RK4 step (builds with adjoints only)
   int (*adj_nslots)(void);
@@ -121,6 +122,11 @@ struct bhah_patch {
   int tile_x = 64, tile_y = 4;
   // adjoint workspace (allocated on the first MoL_step_adjoint call)
   double *d_adj = nullptr;
+  // cached graph of one reverse step (bhah_hip_MoL_step_adjoint) and the arguments it was captured with
+  hipGraphExec_t adj_exec = nullptr;
+  const double *adj_y = nullptr;
+  double *adj_lam = nullptr;
+  double adj_dt = 0;
   // cached graph for MoL_step_n
   hipGraphExec_t graph_exec = nullptr;
   bhah_MoL_gridfunctions graph_gf{};
@@ -153,8 +159,12 @@ struct bhah_patch {
   }
   void invalidate_graph() {
     if (graph_exec) {
-      hipGraphExecDestroy(graph_exec);
+      (void)hipGraphExecDestroy(graph_exec);
       graph_exec = nullptr;
+    }
+    if (adj_exec) {
+      (void)hipGraphExecDestroy(adj_exec);
+      adj_exec = nullptr;
     }
   }
   void all_dims(dim3 &g, dim3 &b) const {
@@ -536,6 +546,36 @@ axpy_kernel(double *__restrict__ dst, const double *__restrict__ src, double a, 
   if (t < n) dst[t] += a * src[t];
 }
 
+// dst[j n + t] = c_j src[t], j = 0..3
+__global__ void __launch_bounds__(256)
+scale4_kernel(double *__restrict__ dst, const double *__restrict__ src, long n, double c0, double c1, double c2,
+              double c3) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const double v = src[t];
+  dst[t] = c0 * v;
+  dst[n + t] = c1 * v;
+  dst[2 * n + t] = c2 * v;
+  dst[3 * n + t] = c3 * v;
+}
+
+// a += x; b += c x
+__global__ void __launch_bounds__(256)
+axpy2_kernel(double *__restrict__ a, double *__restrict__ b, const double *__restrict__ x, double c, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const double v = x[t];
+  a[t] += v;
+  b[t] += c * v;
+}
+
+// out = a + b
+__global__ void __launch_bounds__(256)
+add_kernel(double *__restrict__ out, const double *__restrict__ a, const double *__restrict__ b, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) out[t] = a[t] + b[t];
+}
+
 // ---- Checkpoint payload (write_checkpoint__openmp.c:95-117): point-major copy of y_n.
 __global__ void __launch_bounds__(256)
 checkpoint_pack_kernel(const double *__restrict__ y, double *__restrict__ compact, long ntot, int ngf) {
@@ -553,6 +593,94 @@ checkpoint_unpack_kernel(const int *__restrict__ idx, const double *__restrict__
   const long e = t / ngf;
   const int g = (int)(t - e * ngf);
   y[g * ntot + idx[e]] = compact[t];
+}
+
+// ---- initial_data_reader__convert_ADM_Cartesian_to_BSSN on the device
+// (ADM_Initial_Data_Reader__BSSN_Converter.py:268-460, math ADM_to_BSSN.py:80-160): ADM data in
+// the Cartesian basis at every point (gammaDD00,01,02,11,12,22, KDD.., alpha, betaU, BU) ->
+// BSSN in the Cartesian basis (det ghat = 1 there) -> the patch's spherical-like basis with
+// the Jacobian dx_Cart/dxx of the map -> rescaled variables; lambda^i = 0 here (it is formed
+// by finite differences afterwards).
+__global__ void __launch_bounds__(256)
+adm_cartesian_to_bssn_kernel(const double *__restrict__ adm, double *__restrict__ y, const double *__restrict__ rfm0,
+                             const double *__restrict__ xx1, const double *__restrict__ xx2, int Nx0, int Nx1,
+                             long Ntot) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= Ntot) return;
+  const int i0 = (int)(p % Nx0), i1 = (int)((p / Nx0) % Nx1), i2 = (int)(p / ((long)Nx0 * Nx1));
+  const int sym[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+  double g[3][3], K[3][3];
+  for (int k = 0; k < 6; k++) {
+    const int i = sym[k][0], j = sym[k][1];
+    g[i][j] = g[j][i] = adm[k * Ntot + p];
+    K[i][j] = K[j][i] = adm[(6 + k) * Ntot + p];
+  }
+  auto det3 = [](const double m[3][3]) {
+    return m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) - m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+           m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+  };
+  const double detg = det3(g);
+  double gUU[3][3];
+  gUU[0][0] = (g[1][1] * g[2][2] - g[1][2] * g[2][1]) / detg;
+  gUU[0][1] = gUU[1][0] = (g[0][2] * g[2][1] - g[0][1] * g[2][2]) / detg;
+  gUU[0][2] = gUU[2][0] = (g[0][1] * g[1][2] - g[0][2] * g[1][1]) / detg;
+  gUU[1][1] = (g[0][0] * g[2][2] - g[0][2] * g[2][0]) / detg;
+  gUU[1][2] = gUU[2][1] = (g[0][2] * g[1][0] - g[0][0] * g[1][2]) / detg;
+  gUU[2][2] = (g[0][0] * g[1][1] - g[0][1] * g[1][0]) / detg;
+  // gammabar_ij = (det ghat / det gamma)^(1/3) gamma_ij, W = (det gamma / det gammabar)^(-1/6),
+  // K = gamma^ij K_ij, Abar_ij = (det ghat / det gamma)^(1/3) (K_ij - gamma_ij K / 3)
+  const double fac = cbrt(1.0 / detg);
+  double gb[3][3], Ab[3][3];
+  double trK = 0.0;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) trK += gUU[i][j] * K[i][j];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      gb[i][j] = fac * g[i][j];
+      Ab[i][j] = fac * (K[i][j] - (1.0 / 3.0) * g[i][j] * trK);
+    }
+  const double cf = pow(detg / det3(gb), -1.0 / 6.0);
+  // Jacobian of x = f0(xx0) (sin xx1 cos xx2, sin xx1 sin xx2, cos xx1); its columns are
+  // orthogonal with lengths s = (f0', f0, f0 sin xx1) = 1 / ReU
+  const double f0 = rfm0[i0], f0p = rfm0[Nx0 + i0];
+  const double st = sin(xx1[i1]), ct = cos(xx1[i1]), sp = sin(xx2[i2]), cp = cos(xx2[i2]);
+  const double J[3][3] = {{f0p * st * cp, f0 * ct * cp, -f0 * st * sp},
+                          {f0p * st * sp, f0 * ct * sp, f0 * st * cp},
+                          {f0p * ct, -f0 * st, 0.0}};
+  const double s[3] = {f0p, f0, f0 * st};
+  double gbr[3][3], Abr[3][3];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      double a = 0.0, b = 0.0;
+      for (int m = 0; m < 3; m++)
+        for (int n = 0; n < 3; n++) {
+          a += J[m][i] * gb[m][n] * J[n][j];
+          b += J[m][i] * Ab[m][n] * J[n][j];
+        }
+      gbr[i][j] = a;
+      Abr[i][j] = b;
+    }
+  // evolved order: aDD00..22 (0-5), alpha 6, betU 7-9, cf 10, hDD 11-16, lambdaU 17-19, trK 20, vetU 21-23
+  for (int k = 0; k < 6; k++) {
+    const int i = sym[k][0], j = sym[k][1];
+    const double ReDD = s[i] * s[j];
+    y[(11 + k) * Ntot + p] = (gbr[i][j] - (i == j ? s[i] * s[i] : 0.0)) / ReDD;
+    y[k * Ntot + p] = Abr[i][j] / ReDD;
+  }
+  for (int i = 0; i < 3; i++) {
+    // beta_rfm^i = (J^-1)^i_a beta^a = J[a][i] beta^a / s_i^2; vet^i = beta_rfm^i / ReU^i
+    double vb = 0.0, vB = 0.0;
+    for (int a = 0; a < 3; a++) {
+      vb += J[a][i] * adm[(13 + a) * Ntot + p];
+      vB += J[a][i] * adm[(16 + a) * Ntot + p];
+    }
+    y[(21 + i) * Ntot + p] = vb / s[i];
+    y[(7 + i) * Ntot + p] = vB / s[i];
+    y[(17 + i) * Ntot + p] = 0.0;
+  }
+  y[6 * Ntot + p] = adm[12 * Ntot + p];
+  y[10 * Ntot + p] = cf;
+  y[20 * Ntot + p] = trK;
 }
 
 // ---- Brill-Lindquist initial data on every point (ghost zones included): conformal
@@ -1465,6 +1593,7 @@ static int patch_upload(bhah_patch *P, const std::vector<double> &rfm0, const st
 int bhah_hip_patch_destroy(bhah_patch *P) {
   if (!P) return BHAH_OK;
   if (P->graph_exec) hipGraphExecDestroy(P->graph_exec);
+  if (P->adj_exec) hipGraphExecDestroy(P->adj_exec);
   hipFree(P->d_rfm0);
   hipFree(P->d_rfm1);
   hipFree(P->d_xx0);
@@ -1513,10 +1642,7 @@ int bhah_hip_patch_get_info(const bhah_patch *P, bhah_patch_info *info) {
 int bhah_hip_patch_set_stream(bhah_patch *P, void *s) {
   if (!P) return fail(BHAH_ERR_ARG, "NULL patch");
   P->stream = (hipStream_t)s;
-  if (P->graph_exec) {
-    hipGraphExecDestroy(P->graph_exec);
-    P->graph_exec = nullptr;
-  }
+  P->invalidate_graph();
   return BHAH_OK;
 }
 
@@ -2019,6 +2145,25 @@ extern "C" int bhah_hip_initial_data_brill_lindquist(bhah_patch *P, const double
   return BHAH_OK;
 }
 
+extern "C" int bhah_hip_initial_data_convert_ADM_Cartesian_to_BSSN(bhah_patch *P, const double *adm, double *y_n) {
+  if (!P || !P->k) return fail(BHAH_ERR_ARG, "patch is not a BSSN patch");
+  if (!adm || !y_n) return fail(BHAH_ERR_ARG, "NULL argument");
+  if (P->d.coord == BHAH_CARTESIAN) return fail(BHAH_ERR_UNSUPPORTED, "BSSN patches are spherical-like");
+  dim3 g, b;
+  launch_1d(P->Ntot, g, b);
+  hipLaunchKernelGGL(adm_cartesian_to_bssn_kernel, g, b, 0, P->stream, adm, y_n, P->d_rfm0, P->d_xx1, P->d_xx2,
+                     P->Nx[0], P->Nx[1], P->Ntot);
+  HIP_TRY(hipGetLastError());
+  int rc;
+  // inner boundary points hold parity copies before lambda^i is differenced
+  // (ADM_Initial_Data_Reader__BSSN_Converter.py:986-1010)
+  if ((rc = bhah_hip_apply_bcs_inner_only(P, y_n))) return rc;
+  P->interior_dims(g, b);
+  HIP_TRY(P->k->initial_lambda(g, b, P->stream, y_n, P->d_rfm0, P->d_rfm1, P->kp(0, 0)));
+  // then initial_data() applies the outer extrapolation and inner BCs to everything (initial_data.py:252)
+  return bhah_hip_apply_bcs_outerextrap_and_inner(P, y_n);
+}
+
 // ---------------------------------------------------------------------------
 // Reverse mode of one RK4 step (BSSN, radiation outer BCs).  The step's stages are
 // recomputed from y_n into a patch-owned workspace, then the adjoint runs backwards:
@@ -2052,15 +2197,11 @@ static int adjoint_rhs_bcs(bhah_patch *P, double *lam_k, double *lam_f) {
   return BHAH_OK;
 }
 
-extern "C" int bhah_hip_MoL_step_adjoint(bhah_patch *P, const double *y_n, double dt, double *lam) {
-  if (!P || !P->k || !y_n || !lam) return fail(BHAH_ERR_ARG, "bad arguments");
-  if (!P->k->adj_rhs) return fail(BHAH_ERR_UNSUPPORTED, "kernel build %s has no adjoint kernels", P->k->tag);
-  if (P->d.outer_bc != BHAH_OUTER_BC_RADIATION) return fail(BHAH_ERR_UNSUPPORTED, "adjoint needs radiation outer BCs");
+// Enqueue the reverse of one RK4 step on P->stream (no allocation, no memset: the workspace
+// exists and buffers are cleared by fill kernels, so the sequence can be captured in a graph).
+static int enqueue_adjoint(bhah_patch *P, const double *y_n, double dt, double *lam) {
   const long N = P->Ntot, nv = N * P->ngfs;
   const int nslots = P->k->adj_nslots();
-  // workspace: Y2..Y4 (3 nv), K1..K4 (4 nv), AUX1..4 (4 x 6N), y_run, u, lamK1..4 (4 nv), lamY, lamYN, S
-  const size_t total = (size_t)(3 + 4 + 1 + 1 + 4 + 2) * nv + (size_t)4 * 6 * N + (size_t)nslots * N;
-  if (!P->d_adj) HIP_TRY(hipMalloc((void **)&P->d_adj, total * sizeof(double)));
   double *w = P->d_adj;
   double *Y[4] = {const_cast<double *>(y_n), w, w + nv, w + 2 * nv};
   double *K = w + 3 * nv;  // 4 consecutive arrays
@@ -2068,10 +2209,11 @@ extern "C" int bhah_hip_MoL_step_adjoint(bhah_patch *P, const double *y_n, doubl
   double *AUX = w + 15 * nv;
   double *S = AUX + 24 * N;
   hipStream_t st = P->stream;
-  dim3 ga, ba, gi, bi, g1, b1;
+  dim3 ga, ba, gi, bi, g1, b1, gS, bS;
   P->all_dims(ga, ba);
   P->interior_dims(gi, bi);
   launch_1d(nv, g1, b1);
+  launch_1d((long)nslots * N, gS, bS);
   int rc;
   // ---- forward recompute of the step's stages
   for (int s = 1; s <= 4; s++) {
@@ -2084,26 +2226,26 @@ extern "C" int bhah_hip_MoL_step_adjoint(bhah_patch *P, const double *y_n, doubl
   // ---- reverse
   const double a_stage[4] = {0.5 * dt, 0.5 * dt, dt, 0.0};  // Y[s] = D(y_n + a_stage[s-1] K_s), s = 1..3
   const double b_final[4] = {dt / 6.0, dt / 3.0, dt / 3.0, dt / 6.0};
-  HIP_TRY(hipMemsetAsync(lamK, 0, 4 * nv * sizeof(double), st));
-  // y_{n+1} = D(y_n + sum b K): lam_u = J_D^T lam
+  // y_{n+1} = D(y_n + sum b K): lam_u = J_D^T lam, then lam_K_j = b_j lam_u
   hipLaunchKernelGGL(rk_combine_kernel, g1, b1, 0, st, u, y_n, K, nv, b_final[0], b_final[1], b_final[2], b_final[3]);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(lamYN, lam, nv * sizeof(double), hipMemcpyDeviceToDevice, st));
   HIP_TRY(P->k->adj_det(ga, ba, st, u, lamYN, P->d_rfm0, P->d_rfm1, P->kp(dt, 0)));
-  for (int j = 0; j < 4; j++) {
-    hipLaunchKernelGGL(axpy_kernel, g1, b1, 0, st, lamK + j * nv, lamYN, b_final[j], nv);
-    HIP_TRY(hipGetLastError());
-  }
+  hipLaunchKernelGGL(scale4_kernel, g1, b1, 0, st, lamK, lamYN, nv, b_final[0], b_final[1], b_final[2], b_final[3]);
+  HIP_TRY(hipGetLastError());
   for (int s = 4; s >= 1; s--) {
     double *in = Y[s - 1], *aux = AUX + (s - 1) * 6 * N, *lk = lamK + (s - 1) * nv;
-    HIP_TRY(hipMemsetAsync(lamY, 0, nv * sizeof(double), st));
+    hipLaunchKernelGGL(fill_kernel, g1, b1, 0, st, lamY, 0.0, nv);
+    HIP_TRY(hipGetLastError());
     if ((rc = adjoint_rhs_bcs(P, lk, lamY))) return rc;
-    HIP_TRY(hipMemsetAsync(S, 0, (size_t)nslots * N * sizeof(double), st));
+    hipLaunchKernelGGL(fill_kernel, gS, bS, 0, st, S, 0.0, (long)nslots * N);
+    HIP_TRY(hipGetLastError());
     HIP_TRY(P->k->adj_rhs(gi, bi, st, in, aux, lk, S, P->d_rfm0, P->d_rfm1, P->kp(dt, 0)));
     HIP_TRY(P->k->adj_ricci(gi, bi, st, in, S, P->d_rfm0, P->d_rfm1, P->kp(dt, 0)));
     HIP_TRY(P->k->adj_gather(g1, b1, st, S, lamY, P->kp(dt, 0)));
     if (s == 1) {
-      hipLaunchKernelGGL(axpy_kernel, g1, b1, 0, st, lamYN, lamY, 1.0, nv);
+      // the adjoint of y_n: lamYN + lamY, written straight to the caller's array
+      hipLaunchKernelGGL(add_kernel, g1, b1, 0, st, lam, lamYN, lamY, nv);
       HIP_TRY(hipGetLastError());
       break;
     }
@@ -2113,12 +2255,54 @@ extern "C" int bhah_hip_MoL_step_adjoint(bhah_patch *P, const double *y_n, doubl
     hipLaunchKernelGGL(rk_combine_kernel, g1, b1, 0, st, u, y_n, K, nv, c[0], c[1], c[2], c[3]);
     HIP_TRY(hipGetLastError());
     HIP_TRY(P->k->adj_det(ga, ba, st, u, lamY, P->d_rfm0, P->d_rfm1, P->kp(dt, 0)));
-    hipLaunchKernelGGL(axpy_kernel, g1, b1, 0, st, lamYN, lamY, 1.0, nv);
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(axpy_kernel, g1, b1, 0, st, lamK + (s - 2) * nv, lamY, a_stage[s - 2], nv);
+    hipLaunchKernelGGL(axpy2_kernel, g1, b1, 0, st, lamYN, lamK + (s - 2) * nv, lamY, a_stage[s - 2], nv);
     HIP_TRY(hipGetLastError());
   }
-  HIP_TRY(hipMemcpyAsync(lam, lamYN, nv * sizeof(double), hipMemcpyDeviceToDevice, st));
+  return BHAH_OK;
+}
+
+extern "C" int bhah_hip_MoL_step_adjoint(bhah_patch *P, const double *y_n, double dt, double *lam) {
+  if (!P || !P->k || !y_n || !lam) return fail(BHAH_ERR_ARG, "bad arguments");
+  if (!P->k->adj_rhs) return fail(BHAH_ERR_UNSUPPORTED, "kernel build %s has no adjoint kernels", P->k->tag);
+  if (P->d.outer_bc != BHAH_OUTER_BC_RADIATION) return fail(BHAH_ERR_UNSUPPORTED, "adjoint needs radiation outer BCs");
+  const long N = P->Ntot, nv = N * P->ngfs;
+  const int nslots = P->k->adj_nslots();
+  // workspace: Y2..Y4 (3 nv), K1..K4 (4 nv), AUX1..4 (4 x 6N), y_run, u, lamK1..4 (4 nv), lamY, lamYN, S
+  const size_t total = (size_t)(3 + 4 + 1 + 1 + 4 + 2) * nv + (size_t)4 * 6 * N + (size_t)nslots * N;
+  if (!P->d_adj) HIP_TRY(hipMalloc((void **)&P->d_adj, total * sizeof(double)));
+  // The whole reverse step is one hipGraph, replayed while y_n, lam and dt stay the same (a
+  // caller walking back over stored states copies each into one buffer).
+  if (!(P->adj_exec && P->adj_y == y_n && P->adj_lam == lam && P->adj_dt == dt)) {
+    if (P->adj_exec) {
+      (void)hipGraphExecDestroy(P->adj_exec);
+      P->adj_exec = nullptr;
+    }
+    hipStream_t cap = P->stream;
+    bool tmp_stream = false;
+    if (!cap) {  // the null stream cannot be captured
+      HIP_TRY(hipStreamCreate(&cap));
+      tmp_stream = true;
+    }
+    hipStream_t saved = P->stream;
+    P->stream = cap;
+    hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+    const int rc = e == hipSuccess ? enqueue_adjoint(P, y_n, dt, lam) : BHAH_OK;
+    hipGraph_t graph = nullptr;
+    if (e == hipSuccess) e = hipStreamEndCapture(cap, &graph);
+    P->stream = saved;
+    if (tmp_stream) (void)hipStreamDestroy(cap);
+    if (rc || e != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc ? rc : fail(BHAH_ERR_HIP, "adjoint graph capture failed: %s", hipGetErrorString(e));
+    }
+    e = hipGraphInstantiate(&P->adj_exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(BHAH_ERR_HIP, "adjoint graph instantiate failed: %s", hipGetErrorString(e));
+    P->adj_y = y_n;
+    P->adj_lam = lam;
+    P->adj_dt = dt;
+  }
+  HIP_TRY(hipGraphLaunch(P->adj_exec, P->stream));
   return BHAH_OK;
 }
 

@@ -164,6 +164,7 @@ SIGNATURES: Dict[str, tuple] = {
     "bhah_hip_checkpoint_pack": (_I, [_P, _P, _P]),
     "bhah_hip_checkpoint_unpack": (_I, [_P, _P, ctypes.c_int64, _P]),
     "bhah_hip_initial_data_brill_lindquist": (_I, [_P, ctypes.POINTER(_D), _P]),
+    "bhah_hip_initial_data_convert_ADM_Cartesian_to_BSSN": (_I, [_P, _P, _P]),
     "bhah_hip_ensemble_create": (_I, [ctypes.POINTER(PatchDesc), _I, ctypes.POINTER(_P)]),
     "bhah_hip_ensemble_destroy": (_I, [_P]),
     "bhah_hip_ensemble_size": (_I, [_P]),
@@ -436,6 +437,15 @@ class Patch:
     def initial_data_brill_lindquist(self, y_n, m1, pos1, m2, pos2):
         bh = (ctypes.c_double * 8)(m1, *pos1, m2, *pos2)
         _check(self.lib.bhah_hip_initial_data_brill_lindquist(self.handle, bh, _ptr(y_n)))
+
+    def initial_data_from_adm(self, adm_cartesian, y_n):
+        """initial_data_reader__convert_ADM_Cartesian_to_BSSN + outer extrapolation: adm_cartesian
+        is a (19, Nx2, Nx1, Nx0) float64 device tensor of ADM data in the Cartesian basis
+        (gammaDD00..22, KDD00..22, alpha, betaU0-2, BU0-2) at every point."""
+        if adm_cartesian.numel() < 19 * self.Ntot:
+            raise BHaHError(f"ADM data must hold 19 gridfunctions ({19 * self.Ntot} values)")
+        _check(self.lib.bhah_hip_initial_data_convert_ADM_Cartesian_to_BSSN(self.handle, _ptr(adm_cartesian),
+                                                                           _ptr(y_n)))
 
     def MoL_step_adjoint(self, y_n, dt: float, lam):
         """lam (adjoint of the state after one RK4 step from y_n) <- adjoint of y_n."""

@@ -481,6 +481,15 @@ def rhs_key_to_gf(key: str) -> str:
     return base
 
 
+def initial_lambda_exprs(coord: str) -> Dict[str, sp.Expr]:
+    """lambda^i of converted initial data: Lambdabar^i = gammabar^{jk} (Gammabar^i_jk - Gammahat^i_jk)
+    from finite-difference derivatives of hDD, rescaled by ReU^i
+    (ADM_Initial_Data_Reader__BSSN_Converter.py:462-580, math at :506-523)."""
+    rfm = reference_metric(coord)
+    Bq = BSSNQuantities(rfm, ricci_symbolic=True)
+    return {f"lambdaU{i}": Bq.DGammaU[i] / rfm.ReU[i] for i in range(3)}
+
+
 def detgamma_enforce_exprs(coord: str) -> Dict[str, sp.Expr]:
     """h'_ij = (|det ghat|/det gbar)^{1/3} (ghat_ij/ReDD_ij + h_ij) - ghat_ij/ReDD_ij."""
     rfm = reference_metric(coord)

@@ -220,6 +220,8 @@ static const og_kernel K_RICCI[2] = {KERNEL(ricci_Spherical_sym2_KO0, NULL), KER
 static const og_kernel K_DET[2] = {KERNEL(detgamma_Spherical_sym2_KO0, NULL), KERNEL(detgamma_SinhSpherical_symnone_KO1, NULL)};
 static const og_kernel K_CONS[2] = {KERNEL(constraints_Spherical_sym2_KO0, NULL),
                                     KERNEL(constraints_SinhSpherical_symnone_KO1, NULL)};
+static const og_kernel K_LAM[2] = {KERNEL(initial_lambda_Spherical_sym2_KO0, NULL),
+                                   KERNEL(initial_lambda_SinhSpherical_symnone_KO1, NULL)};
 static const og_kernel K_WAVE = KERNEL(wave_rhs, NULL);
 
 /* Interior ("loop_region=interior") or all-points loop applying a pointwise kernel. */
@@ -265,7 +267,15 @@ void og_enforce_detgamma(const og_grid *g, int variant, const double *rfm0, cons
   free(copy);
 }
 
-/* Constraints H, MU^i (BSSN_constraints.py:82-157) at interior points: out[4*Ntot]. */
+/* lambda^i of converted initial data from finite differences of hDD at interior points, written
+ * into the lambdaU gridfunctions of gfs in place (initial_data_lambdaU_grid_interior,
+ * ADM_Initial_Data_Reader__BSSN_Converter.py:462-580; it reads only hDD and the rfm arrays). */
+void og_initial_lambda(const og_grid *g, int variant, const double *rfm0, const double *rfm1, double *gfs) {
+  run_kernel(&K_LAM[variant], g, rfm0, rfm1, NULL, NULL, gfs, gfs, 0);
+}
+
+/* Constraints H, MU^i, M^2 and |Lambda constraint| (BSSN_constraints.py:82-203) at interior
+ * points: out[6*Ntot]. */
 void og_constraints(const og_grid *g, int variant, const double *rfm0, const double *rfm1, const double *auxevol,
                     const double *in, double *out) {
   run_kernel(&K_CONS[variant], g, rfm0, rfm1, NULL, auxevol, in, out, 0);

@@ -59,6 +59,7 @@ def lib():
             "og_ricci": (None, [P, I, P, P, P, P]),
             "og_enforce_detgamma": (None, [P, I, P, P, P]),
             "og_constraints": (None, [P, I, P, P, P, P, P]),
+            "og_initial_lambda": (None, [P, I, P, P, P]),
             "og_wave_rhs": (None, [P, D, P, P]),
             "og_bcstruct_new": (P, [P, P, P, P]),
             "og_bc_num_inner": (I, [P]),
@@ -178,6 +179,83 @@ class OracleGrid:
                 d[f] = {"integral": I, "L2": np.sqrt(max(I, 0.0)), "RMS": np.sqrt(max(I, 0.0) / vol) if vol > 0 else 0.0}
             out[name] = d
         return out
+
+    def jacobian(self):
+        """dx_Cart^a / dxx^i of the spherical-like map at every point: shape (3, 3, Nx2, Nx1, Nx0)."""
+        P, T, R0 = np.meshgrid(self.xx[2], self.xx[1], self.xx[0], indexing="ij")
+        f0 = np.broadcast_to(self.rfm0[: self.Nx[0]], R0.shape)
+        f0p = np.broadcast_to(self.rfm0[self.Nx[0]: 2 * self.Nx[0]], R0.shape)
+        st, ct, sp_, cp = np.sin(T), np.cos(T), np.sin(P), np.cos(P)
+        J = np.empty((3, 3) + R0.shape)
+        J[:, 0] = f0p * np.stack([st * cp, st * sp_, ct])
+        J[:, 1] = f0 * np.stack([ct * cp, ct * sp_, -st])
+        J[:, 2] = f0 * np.stack([-st * sp_, st * cp, 0 * st])
+        return J
+
+    def adm_cartesian_to_bssn(self, adm) -> np.ndarray:
+        """initial_data_reader__convert_ADM_Cartesian_to_BSSN followed by the outer
+        extrapolation initial_data() applies (ADM_Initial_Data_Reader__BSSN_Converter.py:268-460,
+        1019-1129; initial_data.py:252), restated with numpy and the oracle's own boundary
+        conditions and stencils.  adm: (19, Nx2, Nx1, Nx0), Cartesian basis at every point:
+        gammaDD00,01,02,11,12,22, KDD00..22, alpha, betaU0-2, BU0-2.  Returns the 24 BSSN
+        gridfunctions.
+
+        1. ADM -> BSSN in the Cartesian basis (ADM_to_BSSN.py:80-160 with det(ghat) = 1):
+           gammabar = det(gamma)^(-1/3) gamma, cf = W = (det gamma / det gammabar)^(-1/6),
+           K = gamma^ij K_ij, Abar = det(gamma)^(-1/3) (K_ij - gamma_ij K / 3).
+        2. Cartesian -> reference basis with the Jacobian (covariant indices) and its inverse
+           (contravariant), then rescaling: h = (gammabar - ghat) / ReDD, a = Abar / ReDD,
+           vet = beta / ReU, bet = B / ReU (:344-440).
+        3. lambda^i = 0, inner boundary conditions, then lambda^i by finite differences of h at
+           interior points (:462-580, 986-1010); then extrapolation + inner BCs on everything."""
+        NG = self.NG
+        shp = adm.shape[1:]
+        sym = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+
+        def mat(first):
+            m = np.empty((3, 3) + shp)
+            for k, (i, j) in enumerate(sym):
+                m[i, j] = m[j, i] = adm[first + k]
+            return m
+
+        g, K = mat(0), mat(6)
+        gm = np.moveaxis(g, (0, 1), (-2, -1))
+        detg = np.linalg.det(gm)
+        gbar = g * np.cbrt(1.0 / detg)
+        detgbar = np.linalg.det(np.moveaxis(gbar, (0, 1), (-2, -1)))
+        cf = (detg / detgbar) ** (-1.0 / 6.0)
+        gUU = np.moveaxis(np.linalg.inv(gm), (-2, -1), (0, 1))
+        trK = np.einsum("ij...,ij...->...", gUU, K)
+        Abar = np.cbrt(1.0 / detg) * (K - g * trK / 3.0)
+        J = self.jacobian()
+        Jinv = np.moveaxis(np.linalg.inv(np.moveaxis(J, (0, 1), (-2, -1))), (-2, -1), (0, 1))  # [i, a]
+        gbar_r = np.einsum("ai...,ab...,bj...->ij...", J, gbar, J)
+        Abar_r = np.einsum("ai...,ab...,bj...->ij...", J, Abar, J)
+        beta_r = np.einsum("ia...,a...->i...", Jinv, adm[13:16])
+        B_r = np.einsum("ia...,a...->i...", Jinv, adm[16:19])
+        # scale factors s_i = 1 / ReU^i; ghat_ij = s_i^2 delta_ij; ReDD_ij = s_i s_j
+        P, T, R0 = np.meshgrid(self.xx[2], self.xx[1], self.xx[0], indexing="ij")
+        f0 = np.broadcast_to(self.rfm0[: self.Nx[0]], R0.shape)
+        f0p = np.broadcast_to(self.rfm0[self.Nx[0]: 2 * self.Nx[0]], R0.shape)
+        s = [f0p, f0, f0 * np.sin(T)]
+        y = np.zeros((24,) + shp)
+        E = {n: k for k, n in enumerate(["aDD00", "aDD01", "aDD02", "aDD11", "aDD12", "aDD22", "alpha", "betU0",
+                                         "betU1", "betU2", "cf", "hDD00", "hDD01", "hDD02", "hDD11", "hDD12",
+                                         "hDD22", "lambdaU0", "lambdaU1", "lambdaU2", "trK", "vetU0", "vetU1",
+                                         "vetU2"])}
+        for i, j in sym:
+            ghat = s[i] * s[i] if i == j else 0.0
+            y[E[f"hDD{i}{j}"]] = (gbar_r[i, j] - ghat) / (s[i] * s[j])
+            y[E[f"aDD{i}{j}"]] = Abar_r[i, j] / (s[i] * s[j])
+        for i in range(3):
+            y[E[f"vetU{i}"]] = beta_r[i] * s[i]
+            y[E[f"betU{i}"]] = B_r[i] * s[i]
+        y[E["alpha"]], y[E["cf"]], y[E["trK"]] = adm[12], cf, trK
+        y = np.ascontiguousarray(y)
+        self.apply_bcs_inner_only(y)
+        lib().og_initial_lambda(ctypes.byref(self.g), self.variant, _p(self.rfm0), _p(self.rfm1), _p(y))
+        self.apply_bcs_outerextrap_and_inner(y)
+        return y
 
     def wave_rhs(self, in_gfs, wavespeed=1.0) -> np.ndarray:
         out = self.gfs(2)

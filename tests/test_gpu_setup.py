@@ -17,7 +17,8 @@ CONFIGS = {
                               grid_physical_size=7.5),
     "sinhspherical_192x96x96": dict(coord=2, Nxx=(192, 96, 96), fd_order=6, symmetry_axes=0, enable_KO=True,
                                     grid_physical_size=64.0, SINHW=0.2),
-    "sinhspherical_fd4_odd": dict(coord=2, Nxx=(21, 9, 7), fd_order=4, symmetry_axes=0, enable_KO=True,
+    # odd Nxx0 and Nxx1 (Nxx2 must be even: phi + pi has to be a grid point)
+    "sinhspherical_fd4_odd": dict(coord=2, Nxx=(21, 9, 8), fd_order=4, symmetry_axes=0, enable_KO=True,
                                   grid_physical_size=20.0, SINHW=0.4),
     "ensemble_96x16x4": dict(coord=1, Nxx=(96, 16, 4), fd_order=4, symmetry_axes=4, enable_KO=False,
                              grid_physical_size=10.0),
