@@ -12,10 +12,18 @@ from tests.analytic_fields import adm_data
 pytestmark = pytest.mark.gpu
 
 
+# evolved gridfunctions grouped by tensor: aDD, alpha, betU, cf, hDD, lambdaU, trK, vetU
+GROUPS = [range(0, 6), range(6, 7), range(7, 10), range(10, 11), range(11, 17), range(17, 20), range(20, 21),
+          range(21, 24)]
+LAMBDA = 5
+
+
 def _err(a, b):
+    """Max error of each tensor, relative to that tensor's largest component: a component that
+    is zero in exact arithmetic (betU^theta of a radial B^i) is rounding noise on both sides."""
     num = np.abs(a - b).reshape(a.shape[0], -1).max(axis=1)
-    den = np.maximum(np.abs(b).reshape(b.shape[0], -1).max(axis=1), 1e-6)
-    return num / den
+    den = np.abs(b).reshape(b.shape[0], -1).max(axis=1)
+    return np.array([num[list(g)].max() / max(den[list(g)].max(), 1e-300) for g in GROUPS])
 
 
 @pytest.mark.parametrize("coord,Nxx,fd,sym,KO,variant,axi", [(1, (24, 12, 2), 4, 4, False, 0, True),
@@ -34,10 +42,13 @@ def test_adm_conversion_matches_oracle(hip, oracle_lib, coord, Nxx, fd, sym, KO,
     P.initial_data_from_adm(torch.from_numpy(adm).cuda(), y_d)
     torch.cuda.synchronize()
     er = _err(y_d.cpu().numpy(), y_o)
-    print(f"\nADM -> BSSN {Nxx}: max err {er.max():.2e} (gf {int(er.argmax())}); lambda scale "
+    print(f"\nADM -> BSSN {Nxx}: per-tensor errors {' '.join(f'{e:.1e}' for e in er)}; lambda scale "
           f"{np.abs(y_o[17:20]).max():.2e}")
     assert np.abs(y_o[17:20]).max() > 1e-3
-    assert er.max() < 1e-12, er
+    # algebraic conversion: 1e-12.  lambda^i comes from finite differences of hDD, which
+    # amplify hDD's rounding by sum|w| * invdx (up to 36 here) and 1/ReU (r up to 10): 1e-11.
+    assert np.delete(er, LAMBDA).max() < 1e-12, er
+    assert er[LAMBDA] < 1e-11, er
 
 
 def test_brill_lindquist_adm_data_give_the_closed_form(hip, oracle_lib):
