@@ -302,6 +302,14 @@ def bench_adjoint() -> dict:
     }
 
 
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            return next((line.split(":", 1)[1].strip() for line in f if line.startswith("model name")), "unknown")
+    except OSError:
+        return "unknown"
+
+
 def cpu_baseline(c: dict, min_seconds: float = 10.0, max_steps: int = 8) -> dict:
     """The oracle (plain-C restatement) on the host cores: RK4 steps of the same grid until
     about ``min_seconds`` of CPU work (at most ``max_steps``)."""
@@ -312,7 +320,15 @@ def cpu_baseline(c: dict, min_seconds: float = 10.0, max_steps: int = 8) -> dict
 
     threads = int(os.environ.get("OMP_NUM_THREADS", min(16, os.cpu_count() or 1)))
     os.environ["OMP_NUM_THREADS"] = str(threads)
-    O = oracle.OracleGrid(c["coord"], c["Nxx"], c["fd_order"], c["size"], c["sinhw"], variant=c["variant"])
+    # the restatement built with the reference's project flags (-O2, AVX-512, OpenMP), with
+    # rhs_eval and Ricci_eval unrolled and vectorised along xx0 as the reference's generated
+    # code is; the slot-table checker build where the host has no AVX-512
+    native = oracle.native_supported() and os.path.exists(oracle.NATIVE_LIB_PATH)
+    O = oracle.OracleGrid(c["coord"], c["Nxx"], c["fd_order"], c["size"], c["sinhw"], variant=c["variant"],
+                          native=native)
+    native = native and bool(O.L.og_unrolled_available(c["variant"], c["fd_order"]))
+    form = ("unrolled-stencil C, -O2 -march=x86-64-v4 -fopenmp, xx0 vectorised" if native else
+            "slot-table C, -O2 -march=x86-64-v3 -fopenmp")
     y = ID.brill_lindquist(c["coord"], O.xx, c["size"], c["sinhw"], c["masses"], c["pos1"], c["pos2"])
     O.apply_bcs_outerextrap_and_inner(y)
     st = O.new_state(np.ascontiguousarray(y))
@@ -330,7 +346,8 @@ def cpu_baseline(c: dict, min_seconds: float = 10.0, max_steps: int = 8) -> dict
         "cores": threads,
         "kind": "port",
         "sample": f"{steps} RK4 step(s) of the full {c['Nxx'][0]}x{c['Nxx'][1]}x{c['Nxx'][2]} grid, "
-                  f"plain-C restatement (OpenMP, {threads} threads), {sec:.1f} s",
+                  f"C restatement ({form}; OpenMP, {threads} threads), {sec:.1f} s",
+        "host_cpu": _cpu_model(),
     }
 
 

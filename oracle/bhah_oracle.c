@@ -21,14 +21,6 @@
 
 enum { OG_CARTESIAN = 0, OG_SPHERICAL = 1, OG_SINHSPHERICAL = 2 };
 
-typedef struct {
-  int coord;
-  int Nxx[3], NG, Nx[3];
-  double xxmin[3], xxmax[3], dxx[3], invdxx[3];
-  double p0, p1; /* RMAX | AMPL,SINHW | grid_physical_size */
-  int fd_order;
-} og_grid;
-
 /* numerical_grid_params_Nxx_dxx_xx: nrpy/infrastructures/BHaH/tests/
  * numerical_grids_and_timestep_numerical_grid_params_Nxx_dxx_xx__openmp__HoleySinhSpherical.c:60-104
  * and nrpy/reference_metric.py:1201-1274 (xxmin/xxmax of Spherical, SinhSpherical);
@@ -245,25 +237,75 @@ static void run_kernel(const og_kernel *K, const og_grid *g, const double *rfm0,
 
 /* rhs_eval (nrpy/infrastructures/BHaH/general_relativity/rhs_eval.py:336-379): interior only.
  * params[]: eta, KO_gauge, KO_nongauge, wavespeed. */
+#ifdef OG_UNROLLED
+/* The native-flags build (liboracle_bhah_native.so) runs the unrolled-stencil forms where one
+ * exists for the variant and finite-difference order (OG_UNROLLED_TABLE: kind 0 = rhs_eval,
+ * 1 = Ricci_eval); everything else is shared with the checker build. */
+#include "gen/oracle_unrolled.h"
+
+static og_unrolled_fn unrolled_find(int kind, int variant, int fd_order) {
+  for (size_t k = 0; k < sizeof(OG_UNROLLED_TABLE) / sizeof(OG_UNROLLED_TABLE[0]); k++) {
+    const og_unrolled_entry *e = &OG_UNROLLED_TABLE[k];
+    if (e->kind == kind && e->variant == variant && e->fd_order == fd_order) return e->fn;
+  }
+  return NULL;
+}
+#define unrolled_rhs(v, o) unrolled_find(0, (v), (o))
+#define unrolled_ricci(v, o) unrolled_find(1, (v), (o))
+
+int og_unrolled_available(int variant, int fd_order) {
+  return unrolled_rhs(variant, fd_order) != NULL && unrolled_ricci(variant, fd_order) != NULL;
+}
+#else
+int og_unrolled_available(int variant, int fd_order) {
+  (void)variant;
+  (void)fd_order;
+  return 0;
+}
+#endif
+
 void og_bssn_rhs(const og_grid *g, int variant, const double *params, const double *rfm0, const double *rfm1,
                  const double *auxevol, const double *in, double *rhs) {
+#ifdef OG_UNROLLED
+  const og_unrolled_fn f = unrolled_rhs(variant, g->fd_order);
+  if (f) {
+    f(g, params, rfm0, rfm1, auxevol, in, rhs);
+    return;
+  }
+#endif
   run_kernel(&K_RHS[variant], g, rfm0, rfm1, params, auxevol, in, rhs, 0);
 }
 
 /* Ricci_eval (nrpy/infrastructures/BHaH/general_relativity/Ricci_eval.py:95-142): interior only;
  * writes RbarDD into the 6 auxevol gridfunctions. */
 void og_ricci(const og_grid *g, int variant, const double *rfm0, const double *rfm1, const double *in, double *auxevol) {
+#ifdef OG_UNROLLED
+  const og_unrolled_fn f = unrolled_ricci(variant, g->fd_order);
+  if (f) {
+    f(g, NULL, rfm0, rfm1, NULL, in, auxevol);
+    return;
+  }
+#endif
   run_kernel(&K_RICCI[variant], g, rfm0, rfm1, NULL, NULL, in, auxevol, 0);
 }
 
 /* enforce_detgammabar_equals_detgammahat (…/enforce_detgammabar_equals_detgammahat.py:84-128):
  * all points, in place on hDD. */
 void og_enforce_detgamma(const og_grid *g, int variant, const double *rfm0, const double *rfm1, double *gfs) {
+  const og_kernel *K = &K_DET[variant];
+  /* In place: the kernel reads only its own point's values (no stencil slots) and writes hDD
+   * there after every slot of that point has been read.  A stencil slot would make the update
+   * order-dependent, so evaluate from a copy in that case. */
+  int stencil = 0;
+  for (int k = 0; k < K->nslots; k++) stencil |= K->slots[k].kind == OG_DERIV;
+  if (!stencil) {
+    run_kernel(K, g, rfm0, rfm1, NULL, NULL, gfs, gfs, 1);
+    return;
+  }
   const long Ntot = (long)g->Nx[0] * g->Nx[1] * g->Nx[2];
-  /* evaluate from an unmodified copy of the input, writing hDD back in place */
   double *copy = malloc(sizeof(double) * 24 * Ntot);
   memcpy(copy, gfs, sizeof(double) * 24 * Ntot);
-  run_kernel(&K_DET[variant], g, rfm0, rfm1, NULL, NULL, copy, gfs, 1);
+  run_kernel(K, g, rfm0, rfm1, NULL, NULL, copy, gfs, 1);
   free(copy);
 }
 
@@ -584,6 +626,7 @@ void og_wave_apply_bcs(const og_grid *g, double *gfs) {
 /* RK4 substeps over all points (MoLtimestepping/tests/register_all_MoL_step_forward_in_time.c:4-103). */
 void og_rk_substep(int s, long n, double dt, double *k, double *y_n, double *y_run) {
   const double R16 = 1.0 / 6.0, R12 = 1.0 / 2.0, R13 = 1.0 / 3.0;
+#pragma omp parallel for schedule(static)
   for (long i = 0; i < n; i++) {
     const double kv = k[i], yn = y_n[i];
     if (s == 1) {

@@ -105,6 +105,157 @@ def emit_stencils() -> str:
     return "\n".join(lines)
 
 
+def _lit(w) -> str:
+    """A rational weight as a C constant expression (folded, correctly rounded, at compile time)."""
+    return f"({int(w.p)}.0 / {int(w.q)}.0)" if w.q != 1 else f"{int(w.p)}.0"
+
+
+def _stencil_sum(terms) -> str:
+    """C sum of (weight, expression) terms with the sign folded into the operator."""
+    out = ""
+    for w, e in terms:
+        if not out:
+            out = f"{_lit(w)} * {e}" if w > 0 else f"-{_lit(-w)} * {e}"
+        else:
+            out += f" + {_lit(w)} * {e}" if w > 0 else f" - {_lit(-w)} * {e}"
+    return out
+
+
+def _deriv_c(d, ptr: str, fd_order: int, op: str | None = None) -> str:
+    """Unrolled C expression of one derivative slot at the point ``ptr`` points to, with
+    literal weights and symmetric pairs combined first (the reference's generated form,
+    ``nrpy/finite_difference.py:202-330``: one weight per pair of mirrored points)."""
+    from nrpy_amd.codegen import fd
+
+    op = op or d.op
+    S = ["1", "s1", "s2"]
+    a = d.dirs[0]
+    if op == "dDD" and d.dirs[0] != d.dirs[1]:
+        b = d.dirs[1]
+        w = dict(fd.weights_1d("dD", fd_order))
+        terms = []
+        for i in sorted(o for o in w if o > 0):
+            for j in sorted(o for o in w if o > 0):
+                e = (f"(({ptr}[{i}*{S[a]} + {j}*{S[b]}] - {ptr}[{i}*{S[a]} - {j}*{S[b]}]) - "
+                     f"({ptr}[-{i}*{S[a]} + {j}*{S[b]}] - {ptr}[-{i}*{S[a]} - {j}*{S[b]}]))")
+                terms.append((w[i] * w[j], e))
+        return f"invdx{a} * invdx{b} * ({_stencil_sum(terms)})"
+    w = dict(fd.weights_1d(op, fd_order))
+    scale = f"invdx{a} * invdx{a}" if op == "dDD" else f"invdx{a}"
+    if op in ("dupD", "ddnD"):
+        terms = [(w[o], f"{ptr}[{o}*{S[a]}]" if o else f"{ptr}[0]") for o in sorted(w)]
+        return f"{scale} * ({_stencil_sum(terms)})"
+    pos = sorted(o for o in w if o > 0)
+    if op == "dD":  # antisymmetric: w(-o) = -w(o), no centre weight
+        assert 0 not in w and all(w[-o] == -w[o] for o in pos)
+        terms = [(w[o], f"({ptr}[{o}*{S[a]}] - {ptr}[-{o}*{S[a]}])") for o in pos]
+    else:  # dDD unmixed, dKOD: symmetric
+        assert all(w[-o] == w[o] for o in pos)
+        terms = ([(w[0], f"{ptr}[0]")] if 0 in w else []) + [(w[o], f"({ptr}[{o}*{S[a]}] + {ptr}[-{o}*{S[a]}])")
+                                                             for o in pos]
+    return f"{scale} * ({_stencil_sum(terms)})"
+
+
+def emit_unrolled(spec: specs.Spec, fd_order: int) -> str:
+    """A CPU form of ``{name}_pointwise`` over the interior: every input a named scalar,
+    every stencil unrolled at a fixed finite-difference order, the same CSE'd algebra
+    written out in the loop body, and the xx0 loop vectorised (``omp simd``).  This is
+    the shape of the reference's generated CPU code (``nrpy/infrastructures/BHaH/
+    general_relativity/rhs_eval.py:97-120``: OpenMP over i2, then i1, with SIMD
+    intrinsics along i0 when ``enable_intrinsics`` is set, as in
+    ``nrpy/examples/two_blackholes_collide.py:122``).  It differs from the slot-table
+    form only in the order of the stencil additions and in FMA contraction."""
+    centre, aux, derivs, rfm, params = spec.classify()
+    nm = spec.name
+    used = sorted({spec.gfs.index(d.gf) for d in derivs})
+    body = []
+    for gi in used:
+        body.append(f"const double *restrict f{gi} = in + {gi} * Ntot + idx;")
+    for n in centre:
+        body.append(f"const double {n} = in[{spec.gfs.index(n)} * Ntot + idx];")
+    for n in aux:
+        body.append(f"const double {n} = aux[{spec.aux.index(n)} * Ntot + idx];")
+    for n in rfm:
+        a = specs.RFM_SYMBOLS.index(n)
+        body.append(f"const double {n} = " + (f"rfm0[{a} * s1 + i0];" if a < 4 else f"rfm1_{a - 4};"))
+    for d in derivs:
+        if d.op != "dupD":
+            body.append(f"const double {d.symbol} = {_deriv_c(d, 'f%d' % spec.gfs.index(d.gf), fd_order)};")
+    rats: dict = {}
+    if spec.upwind is not None:
+        # upwinding as the reference's generated code: U = (betaU > 0), U*(up - dn) + dn
+        # (nrpy/c_codegen.py:1085-1150)
+        code, rats = emit.cse_code([f"const double UpwindControlVectorU{i}" for i in range(3)], spec.upwind,
+                                   tmp_prefix="u", indent="", rationals=rats)
+        body.extend(code.split("\n"))
+        for i in range(3):
+            body.append(f"const double Upwind{i} = UpwindControlVectorU{i} > 0.0 ? 1.0 : 0.0;")
+        for d in derivs:
+            if d.op == "dupD":
+                p = "f%d" % spec.gfs.index(d.gf)
+                body.append(f"const double {d.symbol} = Upwind{d.dirs[0]} * ({_deriv_c(d, p, fd_order, 'dupD')} - "
+                            f"{_deriv_c(d, p, fd_order, 'ddnD')}) + {_deriv_c(d, p, fd_order, 'ddnD')};")
+    elif any(d.op == "dupD" for d in derivs):
+        raise ValueError(f"{nm}: upwinded derivatives without an upwind vector")
+    lhs = [f"out[{spec.gfs.index(o) if o in spec.gfs else i} * Ntot + idx]" for i, (o, _) in enumerate(spec.outputs)]
+    code, rats = emit.cse_code(lhs, [e for _, e in spec.outputs], indent="", rationals=rats)
+    body.extend(code.split("\n"))
+    pin = {specs.PARAM_SYMBOLS.index(n): n for n in params}
+    L = [f"void {nm}_fd{fd_order}_unrolled(const og_grid *restrict g, const double *restrict params,",
+         "    const double *restrict rfm0, const double *restrict rfm1, const double *restrict aux,",
+         "    const double *restrict in, double *restrict out) {",
+         "  const long s1 = g->Nx[0], s2 = (long)g->Nx[0] * g->Nx[1], Ntot = s2 * g->Nx[2];",
+         "  const double invdx0 = g->invdxx[0], invdx1 = g->invdxx[1], invdx2 = g->invdxx[2];",
+         "  const int NG = g->NG, N1 = g->Nx[1];",
+         "  (void)aux; (void)params; (void)rfm0; (void)rfm1; (void)N1;",
+         "  (void)invdx0; (void)invdx1; (void)invdx2;",
+         emit.rational_decls(rats)]
+    for k, n in sorted(pin.items()):
+        L.append(f"  const double {n} = params[{k}];")
+    L += ["#pragma omp parallel for schedule(static)",
+          "  for (int i2 = NG; i2 < g->Nx[2] - NG; i2++)",
+          "    for (int i1 = NG; i1 < g->Nx[1] - NG; i1++) {"]
+    for n in rfm:
+        a = specs.RFM_SYMBOLS.index(n)
+        if a >= 4:
+            L.append(f"      const double rfm1_{a - 4} = rfm1[{a - 4} * N1 + i1];")
+    L += ["#pragma omp simd",
+          "      for (int i0 = NG; i0 < g->Nx[0] - NG; i0++) {",
+          "        const long idx = i0 + s1 * i1 + s2 * i2;"]
+    L += ["        " + x.strip() for x in body if x.strip()]
+    L += ["      }", "    }", "}"]
+    return "\n".join(L)
+
+
+# Unrolled CPU forms: (kind: 0 rhs_eval / 1 Ricci_eval, spec builder, variant index, FD order):
+# the two workloads timed on the CPU, SinhSpherical fd6 + KO and Spherical fd4.
+UNROLLED = [(0, specs.bssn_rhs_spec, 1, 6), (1, specs.ricci_spec, 1, 6), (0, specs.bssn_rhs_spec, 0, 4),
+            (1, specs.ricci_spec, 0, 4)]
+
+_GEN_NOTE = ("/* Generated by oracle/gen_oracle.py -- oracle (test) code, do not edit.\n"
+             " * Unrolled-stencil CPU form compiled only into the oracle's native-flags library. */")
+
+
+def emit_unrolled_files() -> dict:
+    """{file name: text}: one translation unit per unrolled form (so make builds them in
+    parallel) and ``oracle_unrolled.h`` with their prototypes and the lookup table."""
+    files = {}
+    table, protos = [], []
+    for kind, f, vi, order in UNROLLED:
+        spec = f(specs.VARIANTS[vi])
+        fn = f"{spec.name}_fd{order}_unrolled"
+        files[f"oracle_unrolled_{spec.name}_fd{order}.c"] = "\n".join(
+            [_GEN_NOTE, "#include <math.h>", '#include "../oracle_types.h"', "", emit_unrolled(spec, order), ""])
+        protos.append(f"void {fn}(const og_grid *restrict, const double *restrict, const double *restrict,\n"
+                      f"    const double *restrict, const double *restrict, const double *restrict, double *restrict);")
+        table.append(f"  {{{kind}, {vi}, {order}, {fn}}},")
+    files["oracle_unrolled.h"] = "\n".join(
+        [_GEN_NOTE, "#pragma once", '#include "../oracle_types.h"', *protos,
+         "typedef struct {", "  int kind, variant, fd_order;", "  og_unrolled_fn fn;", "} og_unrolled_entry;",
+         "static const og_unrolled_entry OG_UNROLLED_TABLE[] = {", *table, "};", ""])
+    return files
+
+
 def main() -> None:
     parts = [
         "/* Generated by oracle/gen_oracle.py -- oracle (test) code, do not edit. */",
@@ -123,6 +274,11 @@ def main() -> None:
     with open(path, "w", encoding="utf-8") as f:
         f.write("\n\n".join(parts) + "\n")
     print("wrote", path)
+    for name, text in emit_unrolled_files().items():
+        path = os.path.join(HERE, "gen", name)
+        with open(path, "w", encoding="utf-8") as f:
+            f.write(text)
+        print("wrote", path)
 
 
 if __name__ == "__main__":

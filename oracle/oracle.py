@@ -16,10 +16,13 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle_bhah.so")
+# the same restatement built with the reference's project flags, with unrolled-stencil
+# rhs_eval / Ricci_eval for the timed workloads (the CPU baseline; never the checker)
+NATIVE_LIB_PATH = os.path.join(HERE, "liboracle_bhah_native.so")
 
 CARTESIAN, SPHERICAL, SINHSPHERICAL = 0, 1, 2
 PARITY = [4, 5, 6, 7, 8, 9, 0, 1, 2, 3, 0, 4, 5, 6, 7, 8, 9, 1, 2, 3, 0, 1, 2, 3]
-_lib = None
+_libs: dict = {}
 
 
 class OGrid(ctypes.Structure):
@@ -38,18 +41,28 @@ class OGrid(ctypes.Structure):
     ]
 
 
-def build() -> str:
-    """Compile the oracle with its Makefile (gcc)."""
-    subprocess.run(["make", "-s", "-C", HERE], check=True)
-    return LIB_PATH
+def build(native: bool = False) -> str:
+    """Compile the oracle with its Makefile (gcc); ``native`` builds the CPU-baseline form."""
+    subprocess.run(["make", "-s", "-C", HERE] + (["-j8", "native"] if native else []), check=True)
+    return NATIVE_LIB_PATH if native else LIB_PATH
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            build()
-        _lib = ctypes.CDLL(LIB_PATH)
+def native_supported() -> bool:
+    """The native form is built for AVX-512 (x86-64-v4): true when this host has it."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            flags = next((line for line in f if line.startswith("flags")), "").split()
+    except OSError:
+        return False
+    return all(x in flags for x in ("avx512f", "avx512dq", "avx512cd", "avx512bw", "avx512vl"))
+
+
+def lib(native: bool = False):
+    path = NATIVE_LIB_PATH if native else LIB_PATH
+    if path not in _libs:
+        if not os.path.exists(path):
+            build(native)
+        _lib = ctypes.CDLL(path)
         P, I, D, L = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_long
         sig = {
             "og_grid_init": (None, [P, I, I, I, I, I, I, D, D]),
@@ -77,12 +90,14 @@ def lib():
             "og_cfl_dt": (D, [P, P, P, P, D]),
             "og_checkpoint_pack": (ctypes.c_long, [ctypes.c_long, I, P, P]),
             "og_checkpoint_unpack": (I, [ctypes.c_long, I, P, P]),
+            "og_unrolled_available": (I, [I, I]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(_lib, name)
             fn.restype = res
             fn.argtypes = args
-    return _lib
+        _libs[path] = _lib
+    return _libs[path]
 
 
 def _p(a: Optional[np.ndarray]):
@@ -96,8 +111,8 @@ class OracleGrid:
     """Grid + rfm arrays + boundary structs of the CPU restatement."""
 
     def __init__(self, coord: int, Nxx, fd_order: int, size: float, sinhw: float = 0.2,
-                 variant: int = 0, NG: Optional[int] = None, with_bc: bool = True):
-        L = lib()
+                 variant: int = 0, NG: Optional[int] = None, with_bc: bool = True, native: bool = False):
+        L = self.L = lib(native)
         self.g = OGrid()
         if NG is None:
             NG = fd_order // 2 + (0 if coord == CARTESIAN else 1)
@@ -116,7 +131,7 @@ class OracleGrid:
 
     def __del__(self):
         if getattr(self, "bc", None):
-            lib().og_bcstruct_free(self.bc, self.NG)
+            self.L.og_bcstruct_free(self.bc, self.NG)
             self.bc = None
 
     def gfs(self, n: int = 24) -> np.ndarray:
@@ -128,24 +143,24 @@ class OracleGrid:
     # ---- pointwise-kernel loops
     def rhs(self, in_gfs, aux, params) -> np.ndarray:
         out = self.gfs(24)
-        lib().og_bssn_rhs(ctypes.byref(self.g), self.variant, _p(params), _p(self.rfm0), _p(self.rfm1),
+        self.L.og_bssn_rhs(ctypes.byref(self.g), self.variant, _p(params), _p(self.rfm0), _p(self.rfm1),
                           _p(aux), _p(in_gfs), _p(out))
         return out
 
     def ricci(self, in_gfs) -> np.ndarray:
         aux = self.gfs(6)
-        lib().og_ricci(ctypes.byref(self.g), self.variant, _p(self.rfm0), _p(self.rfm1), _p(in_gfs), _p(aux))
+        self.L.og_ricci(ctypes.byref(self.g), self.variant, _p(self.rfm0), _p(self.rfm1), _p(in_gfs), _p(aux))
         return aux
 
     def enforce_detgamma(self, gfs) -> np.ndarray:
         out = gfs.copy()
-        lib().og_enforce_detgamma(ctypes.byref(self.g), self.variant, _p(self.rfm0), _p(self.rfm1), _p(out))
+        self.L.og_enforce_detgamma(ctypes.byref(self.g), self.variant, _p(self.rfm0), _p(self.rfm1), _p(out))
         return out
 
     def constraints(self, in_gfs, aux) -> np.ndarray:
         """H, MU0, MU1, MU2, MSQUARED, LAMBDA_CONSTRAINT at interior points (6 gridfunctions)."""
         out = self.gfs(6)
-        lib().og_constraints(ctypes.byref(self.g), self.variant, _p(self.rfm0), _p(self.rfm1), _p(aux),
+        self.L.og_constraints(ctypes.byref(self.g), self.variant, _p(self.rfm0), _p(self.rfm1), _p(aux),
                              _p(in_gfs), _p(out))
         return out
 
@@ -253,18 +268,18 @@ class OracleGrid:
         y[E["alpha"]], y[E["cf"]], y[E["trK"]] = adm[12], cf, trK
         y = np.ascontiguousarray(y)
         self.apply_bcs_inner_only(y)
-        lib().og_initial_lambda(ctypes.byref(self.g), self.variant, _p(self.rfm0), _p(self.rfm1), _p(y))
+        self.L.og_initial_lambda(ctypes.byref(self.g), self.variant, _p(self.rfm0), _p(self.rfm1), _p(y))
         self.apply_bcs_outerextrap_and_inner(y)
         return y
 
     def wave_rhs(self, in_gfs, wavespeed=1.0) -> np.ndarray:
         out = self.gfs(2)
-        lib().og_wave_rhs(ctypes.byref(self.g), wavespeed, _p(in_gfs), _p(out))
+        self.L.og_wave_rhs(ctypes.byref(self.g), wavespeed, _p(in_gfs), _p(out))
         return out
 
     # ---- boundary conditions
     def inner_bc(self):
-        L = lib()
+        L = self.L
         n = L.og_bc_num_inner(self.bc)
         dst, src = np.zeros(n, np.int64), np.zeros(n, np.int64)
         par = np.zeros((n, 10), np.int8)
@@ -273,7 +288,7 @@ class OracleGrid:
         return dst, src, par
 
     def outer_bc(self, gz, d):
-        L = lib()
+        L = self.L
         n = L.og_bc_num_outer(self.bc, gz, d)
         out = np.zeros((n, 6), np.int16)
         if n:
@@ -282,11 +297,11 @@ class OracleGrid:
 
     def apply_bcs_inner_only(self, gfs, parity=None):
         parity = np.array(parity if parity is not None else PARITY[: gfs.shape[0]], dtype=np.int32)
-        lib().og_apply_bcs_inner_only(ctypes.byref(self.g), self.bc, gfs.shape[0], _p(parity), _p(gfs))
+        self.L.og_apply_bcs_inner_only(ctypes.byref(self.g), self.bc, gfs.shape[0], _p(parity), _p(gfs))
 
     def apply_bcs_outerextrap_and_inner(self, gfs, parity=None):
         parity = np.array(parity if parity is not None else PARITY[: gfs.shape[0]], dtype=np.int32)
-        lib().og_apply_bcs_outerextrap_and_inner(ctypes.byref(self.g), self.bc, gfs.shape[0], _p(parity), _p(gfs))
+        self.L.og_apply_bcs_outerextrap_and_inner(ctypes.byref(self.g), self.bc, gfs.shape[0], _p(parity), _p(gfs))
 
     def apply_bcs_outerradiation_and_inner(self, gfs, rhs, rad_order=4):
         parity = np.array(PARITY, dtype=np.int32)
@@ -295,18 +310,18 @@ class OracleGrid:
         ws[6] = np.sqrt(2.0)
         fi[6] = 1.0
         fi[10] = 1.0
-        lib().og_apply_bcs_outerradiation_and_inner(ctypes.byref(self.g), self.bc, _p(self.xx[0]), 24, _p(parity),
+        self.L.og_apply_bcs_outerradiation_and_inner(ctypes.byref(self.g), self.bc, _p(self.xx[0]), 24, _p(parity),
                                                     _p(ws), _p(fi), rad_order, _p(gfs), _p(rhs))
 
     def wave_apply_bcs(self, gfs):
-        lib().og_wave_apply_bcs(ctypes.byref(self.g), _p(gfs))
+        self.L.og_wave_apply_bcs(ctypes.byref(self.g), _p(gfs))
 
     def cfl_dt(self, CFL: float = 0.5) -> float:
-        return lib().og_cfl_dt(ctypes.byref(self.g), *[_p(a) for a in self.xx], CFL)
+        return self.L.og_cfl_dt(ctypes.byref(self.g), *[_p(a) for a in self.xx], CFL)
 
     # ---- time stepping
     def bssn_mol_step(self, state: dict, params, dt: float, rad_order: int = 4) -> None:
-        lib().og_bssn_mol_step(ctypes.byref(self.g), self.variant, _p(params), _p(self.rfm0), _p(self.rfm1),
+        self.L.og_bssn_mol_step(ctypes.byref(self.g), self.variant, _p(params), _p(self.rfm0), _p(self.rfm1),
                                self.bc, _p(self.xx[0]), rad_order, dt, _p(state["y_n"]), _p(state["y_run"]),
                                _p(state["k_odd"]), _p(state["k_even"]), _p(state["aux"]))
 
@@ -320,5 +335,5 @@ class OracleGrid:
         }
 
     def wave_mol_step(self, state: dict, dt: float, wavespeed: float = 1.0) -> None:
-        lib().og_wave_mol_step(ctypes.byref(self.g), wavespeed, dt, _p(state["y_n"]), _p(state["y_run"]),
+        self.L.og_wave_mol_step(ctypes.byref(self.g), wavespeed, dt, _p(state["y_n"]), _p(state["y_run"]),
                                _p(state["k_odd"]), _p(state["k_even"]))

@@ -12,6 +12,21 @@ typedef struct {
   int c, d; /* derivative directions */
 } og_slot;
 
+/* One grid: sizes, ghost width, coordinate bounds and spacings, coordinate parameters. */
+typedef struct {
+  int coord;
+  int Nxx[3], NG, Nx[3];
+  double xxmin[3], xxmax[3], dxx[3], invdxx[3];
+  double p0, p1; /* RMAX | AMPL,SINHW | grid_physical_size */
+  int fd_order;
+} og_grid;
+
+/* Unrolled-stencil form of a pointwise kernel over the interior (native-flags build only):
+ * (grid, params, rfm0, rfm1, aux, in, out). */
+typedef void (*og_unrolled_fn)(const og_grid *restrict, const double *restrict, const double *restrict,
+                               const double *restrict, const double *restrict, const double *restrict,
+                               double *restrict);
+
 /* 1D stencil as exact rationals num/den at integer offsets. */
 typedef struct {
   int n;
