@@ -274,7 +274,9 @@ def bench_adjoint() -> dict:
 
     prob = LapseLossProblem()
     theta0 = [0.5, 0.5, 0.5, -0.5]
-    prob.loss_value(theta0)  # warm-up
+    prob.loss_value(theta0)  # warm-up: graphs, workspaces and the tape's device memory
+    warm = torch.tensor(theta0, dtype=torch.float64, requires_grad=True)
+    prob.loss(warm).backward()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     L = prob.loss_value(theta0)
@@ -291,6 +293,8 @@ def bench_adjoint() -> dict:
     npts = cfg.Nxx[0] * cfg.Nxx[1] * cfg.Nxx[2]
     return {
         "workload": "bssn_lapse_loss_gradient_spherical_96x16x4_400steps_K20",
+        "backward_mode": "all steps taped" if prob.full_tape_fits() else "K-step segments re-run and taped",
+        "tape_GB": cfg.nsteps * prob.P.tape_doubles * 8 / 1e9,
         "loss": L,
         "gradient_m1_m2_z1_z2": [float(x) for x in theta.grad],
         "forward_s": t_fwd,

@@ -295,6 +295,20 @@ int bhah_hip_initial_data_convert_ADM_Cartesian_to_BSSN(bhah_patch *patch, const
  * piecewise constant in beta^i and contributes none. */
 int bhah_hip_MoL_step_adjoint(bhah_patch *patch, const double *y_n, double dt, double *lam);
 
+/* The same reverse step split around a tape (no reference counterpart either), for callers
+ * that keep the forward run's stage data instead of recomputing it: HBM holds many steps'
+ * tapes.  A step's tape is bhah_hip_MoL_step_tape_doubles(patch) doubles of device memory:
+ * the state y_n, the stage inputs Y1..Y3, the right-hand sides K1..K4 after the boundary
+ * conditions and the RbarDD of every stage.
+ * bhah_hip_MoL_step_forward_taped advances y_n by one RK4 step in place, with the kernels and
+ * order of bhah_hip_MoL_step_forward_in_time (the same values bit for bit), and fills the
+ * tape; bhah_hip_MoL_step_adjoint_taped then does what bhah_hip_MoL_step_adjoint does for
+ * that step, reading the tape instead of recomputing the stages.  Both are asynchronous on
+ * the patch's stream and use the patch's adjoint workspace. */
+int64_t bhah_hip_MoL_step_tape_doubles(const bhah_patch *patch);
+int bhah_hip_MoL_step_forward_taped(bhah_patch *patch, double *y_n, double dt, double *tape);
+int bhah_hip_MoL_step_adjoint_taped(bhah_patch *patch, const double *tape, double dt, double *lam);
+
 /* Batched ensembles: many independent BSSN patches of one kernel build, with uneven
  * grids, advanced together.  No single reference counterpart: it batches
  * MoL_step_forward_in_time (MoLtimestepping/MoL_step_forward_in_time.py:39-507) over

@@ -3,10 +3,17 @@
 The loss is the mean squared difference of the lapse alpha sampled at fixed points
 on a sphere r = r_s over the last W of N RK4 steps, against a target run.  The
 forward run is the ordinary MoL evolution (``bhah_hip_MoL_step_forward_in_time``);
-the gradient comes from the hand-written adjoint of one RK4 step
-(``bhah_hip_MoL_step_adjoint``) applied backwards through the run, with the
-states recomputed from checkpoints kept every K steps (memory O(N/K + K) states
-instead of O(N)).  Parameters are the Brill-Lindquist masses and puncture
+the gradient comes from the hand-written adjoint of one RK4 step applied backwards
+through the run.  The reverse step reads a tape of the forward step's stages
+(``bhah_hip_MoL_step_forward_taped`` / ``bhah_hip_MoL_step_adjoint_taped``):
+
+* when the tapes of all N steps fit ``tape_budget_gib`` of device memory, the
+  forward run records them all and the backward pass recomputes nothing;
+* otherwise the forward run keeps the state every K steps, and the backward pass
+  re-runs each K-step segment from its checkpoint, recording that segment's tapes
+  (memory O(N/K) states plus K tapes).
+
+Parameters are the Brill-Lindquist masses and puncture
 positions on the z axis, theta = (m1, m2, z1, z2); the initial data depend on
 them analytically (conformal factor and lapse psi^-2, all else zero), so the
 gradient closes with the exact d y_0 / d theta.
@@ -37,6 +44,7 @@ class LapseLossConfig:
     nsteps: int = 400
     window: int = 50
     checkpoint_every: int = 20
+    tape_budget_gib: float = 32.0
     r_sample: float = 4.0
     n_sample: int = 16
     CFL: float = 0.5
@@ -115,8 +123,10 @@ class LapseLossProblem:
         y0, dy = initial_data(self.xx, theta.detach().cpu().numpy())
         return torch.from_numpy(y0).cuda(), dy
 
-    def _forward(self, theta, record=False, checkpoints: Optional[List] = None):
-        """Run nsteps; returns (loss or None, window samples [W, n])."""
+    def _forward(self, theta, record=False, checkpoints: Optional[List] = None, tape=None):
+        """Run nsteps; returns (loss or None, window samples [W, n]).  With ``tape`` (a
+        [nsteps, tape_doubles] device tensor) every step's stages are recorded; with
+        ``checkpoints`` the state is kept every ``checkpoint_every`` steps."""
         import torch
 
         cfg = self.cfg
@@ -127,7 +137,10 @@ class LapseLossProblem:
         for n in range(1, cfg.nsteps + 1):
             if checkpoints is not None and (n - 1) % cfg.checkpoint_every == 0:
                 checkpoints.append(S.y_n.clone())
-            S.steps(1, self.dt, use_graph=True)
+            if tape is not None:
+                self.P.MoL_step_forward_taped(S.y_n, self.dt, tape[n - 1])
+            else:
+                S.steps(1, self.dt, use_graph=True)
             if n >= first:
                 rec.append(self.samples(S.y_n))
         s = torch.stack(rec)
@@ -141,35 +154,55 @@ class LapseLossProblem:
 
         return float(self._forward(torch.as_tensor(theta, dtype=torch.float64))[0])
 
-    def gradient(self, theta, checkpoints, window_samples):
-        """dL/dtheta by the adjoint recursion over the checkpointed run."""
+    def full_tape_fits(self) -> bool:
+        return self.cfg.nsteps * self.P.tape_doubles * 8 <= self.cfg.tape_budget_gib * 2**30
+
+    def forward_for_gradient(self, theta):
+        """The forward run keeping what the backward pass needs: (loss, window samples, store),
+        store being ("tape", tapes) or ("checkpoints", states)."""
+        import torch
+
+        if self.full_tape_fits():
+            tape = torch.empty((self.cfg.nsteps, self.P.tape_doubles), dtype=torch.float64, device="cuda")
+            L, s = self._forward(theta, tape=tape)
+            return L, s, ("tape", tape)
+        ck: List = []
+        L, s = self._forward(theta, checkpoints=ck)
+        return L, s, ("checkpoints", ck)
+
+    def gradient(self, theta, store, window_samples):
+        """dL/dtheta by the adjoint recursion over the taped (or checkpointed) run."""
         import torch
 
         cfg = self.cfg
-        K = cfg.checkpoint_every
+        kind, data = store
         first = cfg.nsteps - cfg.window + 1
         dLds = 2.0 * (window_samples - self.target_samples) / window_samples.numel()  # [W, n]
-        lam = torch.zeros_like(checkpoints[0])
+        lam = torch.zeros((24,) + tuple(reversed(self.P.Nx)), dtype=torch.float64, device="cuda")
         flat = lam.view(-1)
-        # one state array for the recomputation and one input array for the step adjoint, so
-        # that both run as replays of the graphs captured on their first use
-        R = MoLState(self.P, torch.empty_like(checkpoints[0]))
-        y_in = torch.empty_like(checkpoints[0])
-        for c in range(len(checkpoints) - 1, -1, -1):
-            n0 = c * K
-            n1 = min(n0 + K, cfg.nsteps)
-            # recompute y_{n0} .. y_{n1-1} from the checkpoint
-            states = [checkpoints[c]]
-            R.y_n.copy_(checkpoints[c])
-            for _ in range(n0 + 1, n1):
-                R.steps(1, self.dt, use_graph=True)
-                states.append(R.y_n.clone())
-            for n in range(n1, n0, -1):
-                if n >= first:
-                    g = (dLds[n - first][:, None] * self.s_w).reshape(-1)
-                    flat.index_add_(0, self.s_idx.reshape(-1), g)
-                y_in.copy_(states[n - 1 - n0])
-                self.P.MoL_step_adjoint(y_in, self.dt, lam)
+
+        def reverse(n, tape_n):
+            if n >= first:
+                g = (dLds[n - first][:, None] * self.s_w).reshape(-1)
+                flat.index_add_(0, self.s_idx.reshape(-1), g)
+            self.P.MoL_step_adjoint_taped(tape_n, self.dt, lam)
+
+        if kind == "tape":
+            for n in range(cfg.nsteps, 0, -1):
+                reverse(n, data[n - 1])
+        else:
+            K = cfg.checkpoint_every
+            seg = torch.empty((K, self.P.tape_doubles), dtype=torch.float64, device="cuda")
+            y = torch.empty_like(data[0])
+            for c in range(len(data) - 1, -1, -1):
+                n0 = c * K
+                n1 = min(n0 + K, cfg.nsteps)
+                # re-run steps n0+1 .. n1 from the checkpoint, recording their tapes
+                y.copy_(data[c])
+                for j in range(n1 - n0):
+                    self.P.MoL_step_forward_taped(y, self.dt, seg[j])
+                for n in range(n1, n0, -1):
+                    reverse(n, seg[n - n0 - 1])
         _, dy = self._y0(theta)
         lam_np = lam.cpu().numpy()
         return np.array([np.sum(lam_np * dy[k]) for k in range(4)])
@@ -184,16 +217,16 @@ def _lapse_loss_autograd():
     class LapseLoss(torch.autograd.Function):
         @staticmethod
         def forward(ctx, theta, problem):
-            ck: List = []
-            L, s = problem._forward(theta.detach(), checkpoints=ck)
-            ctx.problem, ctx.ck, ctx.s = problem, ck, s
+            L, s, store = problem.forward_for_gradient(theta.detach())
+            ctx.problem, ctx.store, ctx.s = problem, store, s
             ctx.save_for_backward(theta)
             return L.detach().cpu()
 
         @staticmethod
         def backward(ctx, grad):
             (theta,) = ctx.saved_tensors
-            g = ctx.problem.gradient(theta, ctx.ck, ctx.s)
+            g = ctx.problem.gradient(theta, ctx.store, ctx.s)
+            ctx.store = None  # the tapes are released with the graph
             return grad * torch.from_numpy(g).to(theta.dtype), None
 
     return LapseLoss

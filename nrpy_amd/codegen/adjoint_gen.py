@@ -41,6 +41,7 @@ class SlotTable:
 
     def __init__(self):
         self.slots: List[dict] = []
+        self.aux_slots: set = set()
 
     def add(self, gf: int, taps: List[Tuple[Tuple[int, int, int], sp.Rational, Tuple[int, int, int]]]) -> int:
         """taps: (offset, rational weight, powers of invdxx0..2)."""
@@ -201,38 +202,137 @@ def _slots_for_inputs(spec: specs.Spec, fd_order: int, inputs: Dict[str, str], t
     return lines, aux_slots
 
 
+def _alloc_slots(spec: specs.Spec, fd_order: int, names: List[str], table: "SlotTable") -> Dict[str, tuple]:
+    """One slot per input symbol (two for an upwinded derivative: up, down), as
+    ``_slots_for_inputs`` allocates them; auxiliary inputs are recorded in table.aux_slots."""
+    centre, aux, derivs, rfm, params = spec.classify()
+    dmap = {d.symbol: d for d in derivs}
+    out: Dict[str, tuple] = {}
+    for n in names:
+        if n in centre:
+            out[n] = (table.add(spec.gfs.index(n), [((0, 0, 0), sp.Integer(1), (0, 0, 0))]),)
+        elif n in aux:
+            k = table.add(-1, [])
+            table.aux_slots.add(k)
+            out[n] = (k,)
+        elif n in dmap:
+            d = dmap[n]
+            g = spec.gfs.index(d.gf)
+            if d.op == "dupD":
+                out[n] = (table.add(g, _deriv_taps(d, fd_order, "up")), table.add(g, _deriv_taps(d, fd_order, "dn")))
+            else:
+                out[n] = (table.add(g, _deriv_taps(d, fd_order)),)
+    return out
+
+
+def _deriv_dir(name: str) -> int:
+    return fd.parse_deriv(name).dirs[0]
+
+
+# Output groups of the rhs_eval adjoint.  A group's reverse sweep needs the forward
+# temporaries of its own outputs only, so each is a fraction of the whole sweep (which keeps
+# ~1100 values live and spills); the groups run concurrently.  betU's right-hand side contains
+# lambdaU's (the Gamma-driver), so the three are balanced as {betU}, {lambdaU, vetU, alpha},
+# {the rest}.
+RHS_GROUPS = [("betU",), ("lambdaU", "vetU", "alpha"), ("aDD", "hDD", "cf", "trK")]
+
+
+def _rhs_groups(rspec: specs.Spec) -> List[List[str]]:
+    names = [g for g, _ in rspec.outputs]
+    groups = [[n for n in names if any(n == pre or (n.startswith(pre) and n[len(pre):].isdigit()) for pre in grp)]
+              for grp in RHS_GROUPS]
+    if sorted(sum(groups, [])) != sorted(names):
+        raise ValueError(f"rhs adjoint groups {groups} do not partition the outputs {names}")
+    return [g for g in groups if g]
+
+
 def adjoint_tu(v: specs.Variant, fd_order: int) -> str:
     tag = kernel_name_tag(v, fd_order)
     rats: Dict[str, str] = {}
     pr = emit.DoublePrinter(rats)
     table = SlotTable()
 
-    # ---- rhs_eval adjoint
+    # ---- rhs_eval adjoint, split by output group.  A workgroup is G waves over the same 64
+    # points of an xx0 row, wave g running group g's reverse sweep (each on its own SIMD).
+    # Each wave leaves its input adjoints in LDS; after one barrier the sums over the groups,
+    # in group order, are stored to one slot per input, so the result is deterministic and
+    # the transposed stencils read each input's adjoint once.
     rspec = specs.bssn_rhs_spec(v)
+    groups = _rhs_groups(rspec)
+    G = len(groups)
     lam = {g: f"lam[{bssn.EVOL_GFS.index(g)} * Ntot + idx]" for g, _ in rspec.outputs}
-    fwd, rev, inputs = _reverse_sweep(rspec.outputs, lam, pr, "t")
-    used = set()
-    for _, e in rspec.outputs:
-        used |= {str(s) for s in e.free_symbols}
-    defs = _input_defs(rspec, fd_order, used)
-    stores, aux_slots = _slots_for_inputs(rspec, fd_order, inputs, table, skip_aux=False)
     upw = _upwind_prologue(rspec, fd_order, rats, tables=True)
+    swept = []
+    all_inputs: Dict[str, str] = {}
+    for gi, names in enumerate(groups):
+        outs = [(g, e) for g, e in rspec.outputs if g in names]
+        fwd, rev, inputs = _reverse_sweep(outs, lam, pr, f"g{gi}t")
+        used = set()
+        for _, e in outs:
+            used |= {str(s) for s in e.free_symbols}
+        swept.append((names, _input_defs(rspec, fd_order, used), fwd, rev, inputs))
+        all_inputs.update({n: "" for n in inputs})
+    slot_of = _alloc_slots(rspec, fd_order, sorted(all_inputs), table)
+    aux_slots = {n: [k[0]] for n, k in slot_of.items() if k[0] in table.aux_slots}
+    contrib: Dict[int, List[int]] = {}  # slot -> LDS rows holding its partial sums, group order
+    cases, row = [], 0
+    for gi, (names, defs, fwd, rev, inputs) in enumerate(swept):
+        writes = []
+        for n in sorted(inputs):
+            if n not in slot_of:  # reference-metric values and parameters: constants of the step
+                continue
+            ks, val = slot_of[n], inputs[n]
+            if len(ks) == 2:  # upwinded derivative: up / down slots by the sign of beta^i
+                c = f"(betaU{_deriv_dir(n)} > 0.0)"
+                parts = [(ks[0], f"{c} ? {val} : 0.0"), (ks[1], f"{c} ? 0.0 : {val}")]
+            else:
+                parts = [(ks[0], val)]
+            for k, e in parts:
+                writes.append(f"  red[{row} * 64 + lane] = {e};")
+                contrib.setdefault(k, []).append(row)
+                row += 1
+        body = "\n".join(defs + fwd + rev + writes)
+        cases.append(f"  case {gi}: {{ // {', '.join(names)}\n{body}\n  break;\n  }}")
+    nrows = row
+    store_cases = []
+    slots = sorted(contrib)
+    for gi in range(G):
+        lines = [f"    S[{k} * Ntot + idx] = " + " + ".join(f"red[{r} * 64 + lane]" for r in contrib[k]) + ";"
+                 for j, k in enumerate(slots) if j % G == gi]
+        store_cases.append(f"  case {gi}:\n" + "\n".join(lines) + "\n    break;")
     adj_rhs = f"""
-extern "C" __global__ void __launch_bounds__(256)
+// {G} output groups, one wave each over the same 64 points; {nrows} LDS rows of partial sums
+extern "C" __global__ void __launch_bounds__({64 * G})
 adj_rhs_{tag}(const double *__restrict__ in, const double *__restrict__ aux, const double *__restrict__ lam,
               double *__restrict__ S, const double *__restrict__ rfm0, const double *__restrict__ rfm1,
               const bhah_kparams p) {{
-{PROLOGUE}
+  __shared__ double red[{nrows} * 64];
+  const int lane = (int)threadIdx.x, group = (int)threadIdx.y;
+  const int i0r = p.NG + (int)blockIdx.x * 64 + lane;
+  const bool active = i0r < p.Nx0 - p.NG;
+  const int i0 = active ? i0r : p.Nx0 - p.NG - 1;  // idle lanes repeat the row's last point
+  const int i1 = p.NG + (int)blockIdx.y;
+  const int i2 = p.NG + (int)blockIdx.z;
+  const long s1 = p.s1, s2 = p.s2, Ntot = p.Ntot;
+  const long idx = i0 + s1 * i1 + s2 * i2;
 {upw}
-{chr(10).join(defs)}
-{chr(10).join(fwd)}
-{chr(10).join(rev)}
-{chr(10).join(stores)}
+  switch (group) {{
+{chr(10).join(cases)}
+  default:
+    break;
+  }}
+  __syncthreads();
+  if (!active) return;
+  switch (group) {{
+{chr(10).join(store_cases)}
+  default:
+    break;
+  }}
 }}
 """
     # ---- Ricci_eval adjoint: output adjoints are the auxiliary slots written above
     qspec = specs.ricci_spec(v)
-    qlam = {g: f"S[{aux_slots[g]} * Ntot + idx]" if g in aux_slots else "0.0" for g, _ in qspec.outputs}
+    qlam = {g: f"S[{aux_slots[g][0]} * Ntot + idx]" if g in aux_slots else "0.0" for g, _ in qspec.outputs}
     qfwd, qrev, qinputs = _reverse_sweep(qspec.outputs, qlam, pr, "u")
     qused = set()
     for _, e in qspec.outputs:
@@ -275,60 +375,61 @@ adj_det_{tag}(const double *__restrict__ u, double *__restrict__ lam, const doub
 {chr(10).join(dstore)}
 }}
 """
-    # ---- transposed stencils: table of (gridfunction -> slots -> taps)
+    # ---- transposed stencils, unrolled per gridfunction: lam_y[g](q) += sum over the slots k
+    # of g of scale_k * sum over taps t of w_t S_k(q - off_t).  S is zero outside the interior,
+    # so a tap only needs q - off_t inside the array; its index is clamped into slot k's array
+    # and the value selected away when it lies outside, so every load is unconditional.
     by_gf: Dict[int, List[int]] = {}
     for k, s in enumerate(table.slots):
         if s["gf"] >= 0:
             by_gf.setdefault(s["gf"], []).append(k)
-    gf_start, slot_ids, tap_start, tap_off, tap_w, tap_pw = [], [], [], [], [], []
+    R = max(abs(c) for s in table.slots if s["gf"] >= 0 for off, _, _ in s["taps"] for c in off)
+    cases = []
     for g in range(len(bssn.EVOL_GFS)):
-        gf_start.append(len(slot_ids))
+        body = []
         for k in by_gf.get(g, []):
-            slot_ids.append(k)
-            tap_start.append(len(tap_w))
-            for off, w, pw in table.slots[k]["taps"]:
-                tap_off.append(off)
-                tap_w.append(w)
-                tap_pw.append(pw)
-    gf_start.append(len(slot_ids))
-    tap_start.append(len(tap_w))
-
-    def arr(name, ctype, vals):
-        return f"__constant__ {ctype} {name}[{max(len(vals), 1)}] = {{{', '.join(vals) if vals else '0'}}};"
-
-    tables = "\n".join([
-        arr(f"adjg_gf_start_{tag}", "int", [str(x) for x in gf_start]),
-        arr(f"adjg_slot_{tag}", "int", [str(x) for x in slot_ids]),
-        arr(f"adjg_tap_start_{tag}", "int", [str(x) for x in tap_start]),
-        arr(f"adjg_tap_off_{tag}", "short", [str(c) for o in tap_off for c in o]),
-        arr(f"adjg_tap_w_{tag}", "double", [_dbl(w) for w in tap_w]),
-        arr(f"adjg_tap_pw_{tag}", "signed char", [str(c) for o in tap_pw for c in o]),
-    ])
+            taps = table.slots[k]["taps"]
+            pw = taps[0][2]
+            if any(t[2] != pw for t in taps):
+                raise ValueError(f"slot {k}: taps with different grid-spacing powers")
+            scale = " * ".join([f"inv{d}" for d in range(3) for _ in range(pw[d])]) or "1.0"
+            terms = []
+            for off, w, _ in taps:
+                ok = " & ".join(f"(m{d} >> {off[d] + R})" for d in range(3))
+                lin = f"{off[0]} + {off[1]} * s1 + {off[2]} * s2"
+                terms.append(f"{_dbl(w)} * adjg_tap(S + {k} * Ntot, q - ({lin}), Ntot, ({ok}) & 1u)")
+            body.append(f"      acc += {scale} * (" + "\n          + ".join(terms) + ");")
+        cases.append(f"    case {g}: {{\n" + "\n".join(body) + "\n      break;\n    }")
     gather = f"""
-{tables}
-// lam_y[g](q) += sum over slots k of g, taps t: w_t * invdxx^pw_t * S_k(q - off_t), S_k being zero
-// outside the interior.  One thread per (point, gridfunction); all points.
+__device__ __forceinline__ double adjg_tap(const double *__restrict__ Sk, long j, long Ntot, unsigned ok) {{
+  j = j < 0 ? 0 : (j >= Ntot ? Ntot - 1 : j);
+  const double v = Sk[j];
+  return ok ? v : 0.0;
+}}
+
+// One thread per (point, gridfunction); all points.  Bit o + {R} of m<d> says that the point
+// o steps below along direction d lies inside the array.
 extern "C" __global__ void __launch_bounds__(256)
 adj_gather_{tag}(const double *__restrict__ S, double *__restrict__ lam_y, const bhah_kparams p) {{
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const int g = (int)(t / p.Ntot);
   if (g >= {len(bssn.EVOL_GFS)}) return;
-  const long q = t - (long)g * p.Ntot;
-  const int i0 = (int)(q % p.Nx0), i1 = (int)((q / p.Nx0) % p.Nx1), i2 = (int)(q / p.s2);
-  const double inv[3] = {{p.invdxx0, p.invdxx1, p.invdxx2}};
+  const long Ntot = p.Ntot, s1 = p.s1, s2 = p.s2;
+  const long q = t - (long)g * Ntot;
+  const int i[3] = {{(int)(q % p.Nx0), (int)((q / p.Nx0) % p.Nx1), (int)(q / s2)}};
+  const int Nx[3] = {{p.Nx0, p.Nx1, p.Nx2}};
+  unsigned m[3];
+  for (int d = 0; d < 3; d++) {{
+    m[d] = 0u;
+    for (int o = -{R}; o <= {R}; o++) m[d] |= ((unsigned)(i[d] - o) < (unsigned)Nx[d] ? 1u : 0u) << (o + {R});
+  }}
+  const unsigned m0 = m[0], m1 = m[1], m2 = m[2];
+  const double inv0 = p.invdxx0, inv1 = p.invdxx1, inv2 = p.invdxx2;
   double acc = 0.0;
-  for (int a = adjg_gf_start_{tag}[g]; a < adjg_gf_start_{tag}[g + 1]; a++) {{
-    const int k = adjg_slot_{tag}[a];
-    for (int b = adjg_tap_start_{tag}[a]; b < adjg_tap_start_{tag}[a + 1]; b++) {{
-      const int j0 = i0 - adjg_tap_off_{tag}[3 * b], j1 = i1 - adjg_tap_off_{tag}[3 * b + 1],
-                j2 = i2 - adjg_tap_off_{tag}[3 * b + 2];
-      if (j0 < p.NG || j0 >= p.Nx0 - p.NG || j1 < p.NG || j1 >= p.Nx1 - p.NG || j2 < p.NG || j2 >= p.Nx2 - p.NG)
-        continue;
-      double w = adjg_tap_w_{tag}[b];
-      for (int d = 0; d < 3; d++)
-        for (int e = 0; e < adjg_tap_pw_{tag}[3 * b + d]; e++) w *= inv[d];
-      acc += w * S[(long)k * p.Ntot + j0 + p.s1 * j1 + p.s2 * j2];
-    }}
+  switch (g) {{
+{chr(10).join(cases)}
+    default:
+      break;
   }}
   lam_y[t] += acc;
 }}
@@ -340,6 +441,9 @@ adj_gather_{tag}(const double *__restrict__ S, double *__restrict__ lam_y, const
 extern "C" int adj_nslots_{tag}(void) {{ return {nslots}; }}
 extern "C" hipError_t launch_adj_rhs_{tag}(dim3 g, dim3 b, hipStream_t s, const double *in, const double *aux,
     const double *lam, double *S, const double *rfm0, const double *rfm1, bhah_kparams p) {{
+  // workgroups of 64 points of one xx0 row x {G} output groups
+  g = dim3((unsigned)((p.Nx0 - 2 * p.NG + 63) / 64), (unsigned)(p.Nx1 - 2 * p.NG), (unsigned)(p.Nx2 - 2 * p.NG));
+  b = dim3(64, {G});
   hipLaunchKernelGGL(adj_rhs_{tag}, g, b, 0, s, in, aux, lam, S, rfm0, rfm1, p);
   return hipGetLastError();
 }}

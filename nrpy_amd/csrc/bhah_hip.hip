@@ -107,6 +107,10 @@ struct bhah_patch {
   double *d_rfm0 = nullptr, *d_rfm1 = nullptr, *d_xx0 = nullptr, *d_xx1 = nullptr, *d_xx2 = nullptr;
   int *d_inner_dst = nullptr, *d_inner_src = nullptr;
   unsigned short *d_inner_mask = nullptr;  // bit t set: parity type t flips sign
+  // inner points grouped by source: d_inner_by_src[d_inner_src_start[u] .. [u+1]) lists the
+  // points (indices into the inner arrays) copied from source d_inner_src_list[u]
+  int *d_inner_by_src = nullptr, *d_inner_src_start = nullptr, *d_inner_src_list = nullptr;
+  int n_inner_src = 0;
   int16_t *d_outer[8][3] = {};
   int *d_rad_lines = nullptr;  // (i1, i2) per radial line when the outer points form lines
   int n_rad_lines = 0;
@@ -480,6 +484,7 @@ bc_radiation_adjoint_kernel(const int16_t *__restrict__ pts, int npts, int ngfs,
                             double *__restrict__ lam_f, const double *__restrict__ xx0,
                             const double *__restrict__ wavespeed, RadStencil st, int R, int coord, double AMPL,
                             double SINHW, int Nx0, long s1, long s2, long Ntot, double invdxx0) {
+#pragma clang fp contract(off)
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)npts * ngfs) return;
   const int p = (int)(t % npts), gf = (int)(t / npts);
@@ -509,18 +514,69 @@ bc_radiation_adjoint_kernel(const int16_t *__restrict__ pts, int npts, int ngfs,
   atomicAdd(&lf[idx_int], ratio * c / r_int * lam);
 }
 
-// Reverse of gf[dst] = +-gf[src]: lam[src] += +-lam[dst], lam[dst] = 0.
+// The same reverse for outer points that form radial lines (the layout of
+// bc_radiation_lines_kernel): one thread per (line, gridfunction) walks the layers from the
+// outermost in, with the operations of bc_radiation_adjoint_kernel in the same order.  Only
+// this thread touches its line's values, so the additions need no atomics and the result is
+// the per-layer kernels' bit for bit.
 __global__ void __launch_bounds__(256)
-bc_inner_adjoint_kernel(const int *__restrict__ dst, const int *__restrict__ src,
-                        const unsigned short *__restrict__ mask, int npts, int ngfs,
-                        const int *__restrict__ gf_parity, double *__restrict__ lam, long Ntot) {
+bc_radiation_adjoint_lines_kernel(const int *__restrict__ lines, int nlines, int ngfs, int NG,
+                                  double *__restrict__ lam_rhs, double *__restrict__ lam_f,
+                                  const double *__restrict__ xx0, const double *__restrict__ wavespeed,
+                                  RadStencil st, int R, int coord, double AMPL, double SINHW, int Nx0, long s1,
+                                  long s2, long Ntot, double invdxx0) {
+#pragma clang fp contract(off)
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)npts * ngfs) return;
-  const int p = (int)(t % npts), gf = (int)(t / npts);
+  if (t >= (long)nlines * ngfs) return;
+  const int l = (int)(t % nlines), gf = (int)(t / nlines);
+  const long base = s1 * lines[2 * l] + s2 * lines[2 * l + 1];
+  double *lr = &lam_rhs[gf * Ntot];
+  double *lf = &lam_f[gf * Ntot];
+  const double c = wavespeed[gf];
+  for (int gz = NG - 1; gz >= 0; gz--) {
+    const int i0 = Nx0 - NG + gz, j0 = i0 - 1;
+    const long idx = base + i0, idx_int = base + j0;
+    const double lam = lr[idx];
+    lr[idx] = 0.0;
+    if (lam == 0.0) continue;
+    double r, dxr, r_int, dxr_int;
+    r_and_dx0dr(coord, AMPL, SINHW, xx0[i0], r, dxr);
+    r_and_dx0dr(coord, AMPL, SINHW, xx0[j0], r_int, dxr_int);
+    const double ratio = (r_int * r_int * r_int) / (r * r * r);
+    lr[idx_int] += ratio * lam;
+    long at[8];
+    double w[8];
+    int n = partial_r_taps(st, R, Nx0, idx, i0, -1, at, w);
+    for (int j = 0; j < n; j++) lf[at[j]] += -c * dxr * invdxx0 * w[j] * lam;
+    lf[idx] += -c / r * lam;
+    n = partial_r_taps(st, R, Nx0, idx_int, j0, -1, at, w);
+    for (int j = 0; j < n; j++) lf[at[j]] += ratio * c * dxr_int * invdxx0 * w[j] * lam;
+    lf[idx_int] += ratio * c / r_int * lam;
+  }
+}
+
+// Reverse of gf[dst] = +-gf[src]: lam[src] += sum of +-lam[dst] over the points copied from
+// src, lam[dst] = 0.  One thread per (source, gridfunction), summing its points in a fixed
+// order, so the result does not depend on scheduling.  Sources are never destinations, and
+// every destination belongs to one source, so no two threads touch the same value.
+__global__ void __launch_bounds__(256)
+bc_inner_adjoint_kernel(const int *__restrict__ dst, const int *__restrict__ by_src,
+                        const int *__restrict__ src_start, const int *__restrict__ src_list, int nsrc,
+                        const unsigned short *__restrict__ mask, int ngfs, const int *__restrict__ gf_parity,
+                        double *__restrict__ lam, long Ntot) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)nsrc * ngfs) return;
+  const int u = (int)(t % nsrc), gf = (int)(t / nsrc);
   double *l = &lam[gf * Ntot];
-  const double v = l[dst[p]];
-  l[dst[p]] = 0.0;
-  if (v != 0.0) atomicAdd(&l[src[p]], ((mask[p] >> gf_parity[gf]) & 1u) ? -v : v);
+  const int par = gf_parity[gf];
+  double acc = 0.0;
+  for (int j = src_start[u]; j < src_start[u + 1]; j++) {
+    const int p = by_src[j];
+    const double v = l[dst[p]];
+    l[dst[p]] = 0.0;
+    acc += ((mask[p] >> par) & 1u) ? -v : v;
+  }
+  l[src_list[u]] += acc;
 }
 
 // out = y + c0 k0 + c1 k1 + c2 k2 + c3 k3 over n values (RK stage arguments).
@@ -1543,11 +1599,24 @@ static int patch_upload(bhah_patch *P, const std::vector<double> &rfm0, const st
     fi[10] = 1.0;       // cf
   }
   std::vector<int> par(kBssnParity, kBssnParity + 24);
+  // The inner points grouped by source (sources ascending, each group's points in list order):
+  // the adjoint of the copies gathers per source in this fixed order, so it is deterministic.
+  std::vector<int> order(dst.size()), by_src_start, by_src_src;
+  for (size_t p = 0; p < order.size(); p++) order[p] = (int)p;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return src[a] < src[b]; });
+  for (size_t j = 0; j < order.size(); j++)
+    if (j == 0 || src[order[j]] != src[order[j - 1]]) {
+      by_src_start.push_back((int)j);
+      by_src_src.push_back(src[order[j]]);
+    }
+  by_src_start.push_back((int)order.size());
+  P->n_inner_src = (int)by_src_src.size();
   if ((rc = upload(rfm0, &P->d_rfm0)) || (rc = upload(rfm1, &P->d_rfm1)) || (rc = upload(P->xx[0], &P->d_xx0)) ||
       (rc = upload(P->xx[1], &P->d_xx1)) || (rc = upload(P->xx[2], &P->d_xx2)) ||
       (rc = upload(dst, &P->d_inner_dst)) || (rc = upload(src, &P->d_inner_src)) ||
       (rc = upload(mask, &P->d_inner_mask)) || (rc = upload(ws, &P->d_wavespeed)) || (rc = upload(fi, &P->d_finf)) ||
-      (rc = upload(par, &P->d_gf_parity))) {
+      (rc = upload(par, &P->d_gf_parity)) || (rc = upload(order, &P->d_inner_by_src)) ||
+      (rc = upload(by_src_start, &P->d_inner_src_start)) || (rc = upload(by_src_src, &P->d_inner_src_list))) {
     bhah_hip_patch_destroy(P);
     return rc;
   }
@@ -1600,6 +1669,9 @@ int bhah_hip_patch_destroy(bhah_patch *P) {
   hipFree(P->d_xx1);
   hipFree(P->d_xx2);
   hipFree(P->d_inner_dst);
+  hipFree(P->d_inner_by_src);
+  hipFree(P->d_inner_src_start);
+  hipFree(P->d_inner_src_list);
   hipFree(P->d_inner_src);
   hipFree(P->d_inner_mask);
   hipFree(P->d_wavespeed);
@@ -2172,17 +2244,27 @@ extern "C" int bhah_hip_initial_data_convert_ADM_Cartesian_to_BSSN(bhah_patch *P
 
 static int adjoint_rhs_bcs(bhah_patch *P, double *lam_k, double *lam_f) {
   // forward order: radiation layers gz = 0..NG-1 (directions 0..2), then inner parity
-  const int n_in = (int)P->inner_dst.size();
-  if (n_in) {
+  if (P->n_inner_src) {
     dim3 g, b;
-    launch_1d((long)n_in * P->ngfs, g, b);
-    hipLaunchKernelGGL(bc_inner_adjoint_kernel, g, b, 0, P->stream, P->d_inner_dst, P->d_inner_src, P->d_inner_mask,
-                       n_in, P->ngfs, P->d_gf_parity, lam_k, P->Ntot);
+    launch_1d((long)P->n_inner_src * P->ngfs, g, b);
+    hipLaunchKernelGGL(bc_inner_adjoint_kernel, g, b, 0, P->stream, P->d_inner_dst, P->d_inner_by_src,
+                       P->d_inner_src_start, P->d_inner_src_list, P->n_inner_src, P->d_inner_mask, P->ngfs,
+                       P->d_gf_parity, lam_k, P->Ntot);
     HIP_TRY(hipGetLastError());
   }
   const int R = (P->d.radiation_bc_fd_order > 0 ? P->d.radiation_bc_fd_order : 4) / 2;
   RadStencil st;
   rad_stencil(R, st);
+  if (P->n_rad_lines) {  // outer points on radial lines: every layer in one launch
+    dim3 g, b;
+    launch_1d((long)P->n_rad_lines * P->ngfs, g, b);
+    hipLaunchKernelGGL(bc_radiation_adjoint_lines_kernel, g, b, 0, P->stream, P->d_rad_lines, P->n_rad_lines,
+                       P->ngfs, P->NG, lam_k, lam_f, P->d_xx0, P->d_wavespeed, st, R, P->d.coord,
+                       P->d.grid_physical_size, P->d.SINHW, P->Nx[0], (long)P->Nx[0], (long)P->Nx[0] * P->Nx[1],
+                       P->Ntot, P->invdxx[0]);
+    HIP_TRY(hipGetLastError());
+    return BHAH_OK;
+  }
   for (int gz = P->NG - 1; gz >= 0; gz--)
     for (int d = 2; d >= 0; d--) {
       const int n = (int)(P->outer[gz][d].size() / 6);
@@ -2197,79 +2279,164 @@ static int adjoint_rhs_bcs(bhah_patch *P, double *lam_k, double *lam_f) {
   return BHAH_OK;
 }
 
-// Enqueue the reverse of one RK4 step on P->stream (no allocation, no memset: the workspace
-// exists and buffers are cleared by fill kernels, so the sequence can be captured in a graph).
-static int enqueue_adjoint(bhah_patch *P, const double *y_n, double dt, double *lam) {
+// A step's tape: the stage data the reverse needs, in one block of 8 * ngfs * Ntot +
+// 4 * 6 * Ntot doubles: Y0 (the state y_n), Y1..Y3 (stage inputs), K1..K4 (right-hand sides
+// after the BCs), AUX1..AUX4 (RbarDD of each stage).
+struct StepTape {
+  double *Y[4], *K, *AUX;
+};
+
+static StepTape tape_view(const bhah_patch *P, double *t) {
   const long N = P->Ntot, nv = N * P->ngfs;
-  const int nslots = P->k->adj_nslots();
-  double *w = P->d_adj;
-  double *Y[4] = {const_cast<double *>(y_n), w, w + nv, w + 2 * nv};
-  double *K = w + 3 * nv;  // 4 consecutive arrays
-  double *yrun = w + 7 * nv, *u = w + 8 * nv, *lamK = w + 9 * nv, *lamY = w + 13 * nv, *lamYN = w + 14 * nv;
-  double *AUX = w + 15 * nv;
-  double *S = AUX + 24 * N;
-  hipStream_t st = P->stream;
-  dim3 ga, ba, gi, bi, g1, b1, gS, bS;
+  StepTape T;
+  for (int s = 0; s < 4; s++) T.Y[s] = t + s * nv;
+  T.K = t + 4 * nv;
+  T.AUX = t + 8 * nv;
+  return T;
+}
+
+static long tape_doubles(const bhah_patch *P) { return (8L * P->ngfs + 4L * 6) * P->Ntot; }
+
+// Enqueue the stages of one RK4 step from T.Y[0] into the tape, the kernels and their order
+// being those of the MoL step.  With y_next, the step's last update (rk_det of stage 4) also
+// writes y_{n+1} there, as the MoL step writes it into y_n.
+static int enqueue_stages(bhah_patch *P, const StepTape &T, double *yrun, double dt, double *y_next) {
+  const long N = P->Ntot, nv = N * P->ngfs;
+  dim3 ga, ba;
   P->all_dims(ga, ba);
-  P->interior_dims(gi, bi);
-  launch_1d(nv, g1, b1);
-  launch_1d((long)nslots * N, gS, bS);
   int rc;
-  // ---- forward recompute of the step's stages
   for (int s = 1; s <= 4; s++) {
-    double *in = Y[s - 1], *k = K + (s - 1) * nv, *aux = AUX + (s - 1) * 6 * N;
+    double *in = T.Y[s - 1], *k = T.K + (s - 1) * nv, *aux = T.AUX + (s - 1) * 6 * N;
     if ((rc = bhah_hip_Ricci_eval(P, in, aux))) return rc;
     if ((rc = bhah_hip_rhs_eval(P, aux, in, k))) return rc;
     if ((rc = bhah_hip_apply_bcs_outerradiation_and_inner(P, in, k))) return rc;
-    if (s < 4) HIP_TRY(P->k->rk_det(ga, ba, st, k, y_n, yrun, Y[s], P->d_rfm0, P->d_rfm1, P->kp(dt, s)));
+    if (s < 4)
+      HIP_TRY(P->k->rk_det(ga, ba, P->stream, k, T.Y[0], yrun, T.Y[s], P->d_rfm0, P->d_rfm1, P->kp(dt, s)));
+    else if (y_next)
+      HIP_TRY(P->k->rk_det(ga, ba, P->stream, k, T.Y[0], yrun, y_next, P->d_rfm0, P->d_rfm1, P->kp(dt, s)));
   }
-  // ---- reverse
+  return BHAH_OK;
+}
+
+// Workspace of the reverse sweep, after the recomputed tape and y_run in P->d_adj:
+// u, lamK1..4, lamY, lamYN (7 nv) and the adjoint slots S (nslots * Ntot, zero at every
+// non-interior point: filled once when the workspace is made, then written at interior points
+// only, every slot at every point, by adj_rhs and adj_ricci).
+struct AdjWork {
+  double *u, *lamK, *lamY, *lamYN, *S;
+};
+
+static AdjWork adj_work(const bhah_patch *P) {
+  const long nv = P->Ntot * P->ngfs;
+  double *w = P->d_adj + tape_doubles(P) + nv;
+  return {w, w + nv, w + 5 * nv, w + 6 * nv, w + 7 * nv};
+}
+
+static int ensure_adj_workspace(bhah_patch *P) {
+  if (P->d_adj) return BHAH_OK;
+  const long N = P->Ntot, nv = N * P->ngfs;
+  const int nslots = P->k->adj_nslots();
+  const size_t total = (size_t)tape_doubles(P) + (size_t)nv + 7 * (size_t)nv + (size_t)nslots * N;
+  HIP_TRY(hipMalloc((void **)&P->d_adj, total * sizeof(double)));
+  dim3 g, b;
+  launch_1d((long)nslots * N, g, b);
+  hipLaunchKernelGGL(fill_kernel, g, b, 0, P->stream, adj_work(P).S, 0.0, (long)nslots * N);
+  HIP_TRY(hipGetLastError());
+  return BHAH_OK;
+}
+
+// Enqueue the reverse of one RK4 step from its tape on P->stream (no allocation, no memset, so
+// the sequence can be captured in a graph).
+static int enqueue_reverse(bhah_patch *P, const StepTape &T, double dt, double *lam) {
+  const long N = P->Ntot, nv = N * P->ngfs;
+  const AdjWork W = adj_work(P);
+  const double *y_n = T.Y[0];
+  const double *K = T.K;
+  hipStream_t st = P->stream;
+  dim3 ga, ba, gi, bi, g1, b1;
+  P->all_dims(ga, ba);
+  P->interior_dims(gi, bi);
+  launch_1d(nv, g1, b1);
+  int rc;
   const double a_stage[4] = {0.5 * dt, 0.5 * dt, dt, 0.0};  // Y[s] = D(y_n + a_stage[s-1] K_s), s = 1..3
   const double b_final[4] = {dt / 6.0, dt / 3.0, dt / 3.0, dt / 6.0};
   // y_{n+1} = D(y_n + sum b K): lam_u = J_D^T lam, then lam_K_j = b_j lam_u
-  hipLaunchKernelGGL(rk_combine_kernel, g1, b1, 0, st, u, y_n, K, nv, b_final[0], b_final[1], b_final[2], b_final[3]);
+  hipLaunchKernelGGL(rk_combine_kernel, g1, b1, 0, st, W.u, y_n, K, nv, b_final[0], b_final[1], b_final[2],
+                     b_final[3]);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipMemcpyAsync(lamYN, lam, nv * sizeof(double), hipMemcpyDeviceToDevice, st));
-  HIP_TRY(P->k->adj_det(ga, ba, st, u, lamYN, P->d_rfm0, P->d_rfm1, P->kp(dt, 0)));
-  hipLaunchKernelGGL(scale4_kernel, g1, b1, 0, st, lamK, lamYN, nv, b_final[0], b_final[1], b_final[2], b_final[3]);
+  HIP_TRY(hipMemcpyAsync(W.lamYN, lam, nv * sizeof(double), hipMemcpyDeviceToDevice, st));
+  HIP_TRY(P->k->adj_det(ga, ba, st, W.u, W.lamYN, P->d_rfm0, P->d_rfm1, P->kp(dt, 0)));
+  hipLaunchKernelGGL(scale4_kernel, g1, b1, 0, st, W.lamK, W.lamYN, nv, b_final[0], b_final[1], b_final[2],
+                     b_final[3]);
   HIP_TRY(hipGetLastError());
   for (int s = 4; s >= 1; s--) {
-    double *in = Y[s - 1], *aux = AUX + (s - 1) * 6 * N, *lk = lamK + (s - 1) * nv;
-    hipLaunchKernelGGL(fill_kernel, g1, b1, 0, st, lamY, 0.0, nv);
+    const double *in = T.Y[s - 1], *aux = T.AUX + (s - 1) * 6 * N;
+    double *lk = W.lamK + (s - 1) * nv;
+    hipLaunchKernelGGL(fill_kernel, g1, b1, 0, st, W.lamY, 0.0, nv);
     HIP_TRY(hipGetLastError());
-    if ((rc = adjoint_rhs_bcs(P, lk, lamY))) return rc;
-    hipLaunchKernelGGL(fill_kernel, gS, bS, 0, st, S, 0.0, (long)nslots * N);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(P->k->adj_rhs(gi, bi, st, in, aux, lk, S, P->d_rfm0, P->d_rfm1, P->kp(dt, 0)));
-    HIP_TRY(P->k->adj_ricci(gi, bi, st, in, S, P->d_rfm0, P->d_rfm1, P->kp(dt, 0)));
-    HIP_TRY(P->k->adj_gather(g1, b1, st, S, lamY, P->kp(dt, 0)));
+    if ((rc = adjoint_rhs_bcs(P, lk, W.lamY))) return rc;
+    HIP_TRY(P->k->adj_rhs(gi, bi, st, in, aux, lk, W.S, P->d_rfm0, P->d_rfm1, P->kp(dt, 0)));
+    HIP_TRY(P->k->adj_ricci(gi, bi, st, in, W.S, P->d_rfm0, P->d_rfm1, P->kp(dt, 0)));
+    HIP_TRY(P->k->adj_gather(g1, b1, st, W.S, W.lamY, P->kp(dt, 0)));
     if (s == 1) {
       // the adjoint of y_n: lamYN + lamY, written straight to the caller's array
-      hipLaunchKernelGGL(add_kernel, g1, b1, 0, st, lam, lamYN, lamY, nv);
+      hipLaunchKernelGGL(add_kernel, g1, b1, 0, st, lam, W.lamYN, W.lamY, nv);
       HIP_TRY(hipGetLastError());
       break;
     }
     // Y[s-1] = D(u), u = y_n + a K_{s-1}
     double c[4] = {0, 0, 0, 0};
     c[s - 2] = a_stage[s - 2];
-    hipLaunchKernelGGL(rk_combine_kernel, g1, b1, 0, st, u, y_n, K, nv, c[0], c[1], c[2], c[3]);
+    hipLaunchKernelGGL(rk_combine_kernel, g1, b1, 0, st, W.u, y_n, K, nv, c[0], c[1], c[2], c[3]);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(P->k->adj_det(ga, ba, st, u, lamY, P->d_rfm0, P->d_rfm1, P->kp(dt, 0)));
-    hipLaunchKernelGGL(axpy2_kernel, g1, b1, 0, st, lamYN, lamK + (s - 2) * nv, lamY, a_stage[s - 2], nv);
+    HIP_TRY(P->k->adj_det(ga, ba, st, W.u, W.lamY, P->d_rfm0, P->d_rfm1, P->kp(dt, 0)));
+    hipLaunchKernelGGL(axpy2_kernel, g1, b1, 0, st, W.lamYN, W.lamK + (s - 2) * nv, W.lamY, a_stage[s - 2], nv);
     HIP_TRY(hipGetLastError());
   }
   return BHAH_OK;
 }
 
-extern "C" int bhah_hip_MoL_step_adjoint(bhah_patch *P, const double *y_n, double dt, double *lam) {
-  if (!P || !P->k || !y_n || !lam) return fail(BHAH_ERR_ARG, "bad arguments");
+// The reverse of one step from y_n alone: recompute the tape into the workspace, then reverse.
+static int enqueue_adjoint(bhah_patch *P, const double *y_n, double dt, double *lam) {
+  StepTape T = tape_view(P, P->d_adj);
+  T.Y[0] = const_cast<double *>(y_n);  // read only: the stages are written to Y1..Y3, K, AUX
+  int rc;
+  if ((rc = enqueue_stages(P, T, P->d_adj + tape_doubles(P), dt, nullptr))) return rc;
+  return enqueue_reverse(P, T, dt, lam);
+}
+
+static int check_adjoint_patch(const bhah_patch *P) {
   if (!P->k->adj_rhs) return fail(BHAH_ERR_UNSUPPORTED, "kernel build %s has no adjoint kernels", P->k->tag);
   if (P->d.outer_bc != BHAH_OUTER_BC_RADIATION) return fail(BHAH_ERR_UNSUPPORTED, "adjoint needs radiation outer BCs");
-  const long N = P->Ntot, nv = N * P->ngfs;
-  const int nslots = P->k->adj_nslots();
-  // workspace: Y2..Y4 (3 nv), K1..K4 (4 nv), AUX1..4 (4 x 6N), y_run, u, lamK1..4 (4 nv), lamY, lamYN, S
-  const size_t total = (size_t)(3 + 4 + 1 + 1 + 4 + 2) * nv + (size_t)4 * 6 * N + (size_t)nslots * N;
-  if (!P->d_adj) HIP_TRY(hipMalloc((void **)&P->d_adj, total * sizeof(double)));
+  return BHAH_OK;
+}
+
+extern "C" int64_t bhah_hip_MoL_step_tape_doubles(const bhah_patch *P) {
+  if (!P || !P->k) return -1;
+  return tape_doubles(P);
+}
+
+extern "C" int bhah_hip_MoL_step_forward_taped(bhah_patch *P, double *y_n, double dt, double *tape) {
+  if (!P || !P->k || !y_n || !tape) return fail(BHAH_ERR_ARG, "bad arguments");
+  int rc;
+  if ((rc = check_adjoint_patch(P)) || (rc = ensure_adj_workspace(P))) return rc;
+  const long nv = P->Ntot * P->ngfs;
+  StepTape T = tape_view(P, tape);
+  HIP_TRY(hipMemcpyAsync(T.Y[0], y_n, nv * sizeof(double), hipMemcpyDeviceToDevice, P->stream));
+  return enqueue_stages(P, T, P->d_adj + tape_doubles(P), dt, y_n);
+}
+
+extern "C" int bhah_hip_MoL_step_adjoint_taped(bhah_patch *P, const double *tape, double dt, double *lam) {
+  if (!P || !P->k || !tape || !lam) return fail(BHAH_ERR_ARG, "bad arguments");
+  int rc;
+  if ((rc = check_adjoint_patch(P)) || (rc = ensure_adj_workspace(P))) return rc;
+  return enqueue_reverse(P, tape_view(P, const_cast<double *>(tape)), dt, lam);
+}
+
+extern "C" int bhah_hip_MoL_step_adjoint(bhah_patch *P, const double *y_n, double dt, double *lam) {
+  if (!P || !P->k || !y_n || !lam) return fail(BHAH_ERR_ARG, "bad arguments");
+  int rc0;
+  if ((rc0 = check_adjoint_patch(P)) || (rc0 = ensure_adj_workspace(P))) return rc0;
   // The whole reverse step is one hipGraph, replayed while y_n, lam and dt stay the same (a
   // caller walking back over stored states copies each into one buffer).
   if (!(P->adj_exec && P->adj_y == y_n && P->adj_lam == lam && P->adj_dt == dt)) {

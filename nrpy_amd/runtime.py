@@ -160,6 +160,9 @@ SIGNATURES: Dict[str, tuple] = {
     "bhah_hip_diagnostic_gfs_malloc": (_I, [_P, ctypes.POINTER(_P)]),
     "bhah_hip_diagnostic_gfs_free": (_I, [_P]),
     "bhah_hip_MoL_step_adjoint": (_I, [_P, _P, _D, _P]),
+    "bhah_hip_MoL_step_tape_doubles": (ctypes.c_int64, [_P]),
+    "bhah_hip_MoL_step_forward_taped": (_I, [_P, _P, _D, _P]),
+    "bhah_hip_MoL_step_adjoint_taped": (_I, [_P, _P, _D, _P]),
     "bhah_hip_checkpoint_payload_bytes": (ctypes.c_int64, [_P]),
     "bhah_hip_checkpoint_pack": (_I, [_P, _P, _P]),
     "bhah_hip_checkpoint_unpack": (_I, [_P, _P, ctypes.c_int64, _P]),
@@ -450,6 +453,27 @@ class Patch:
     def MoL_step_adjoint(self, y_n, dt: float, lam):
         """lam (adjoint of the state after one RK4 step from y_n) <- adjoint of y_n."""
         _check(self.lib.bhah_hip_MoL_step_adjoint(self.handle, _ptr(y_n), dt, _ptr(lam)))
+
+    @property
+    def tape_doubles(self) -> int:
+        """Doubles of one step's tape (bhah_hip_MoL_step_tape_doubles)."""
+        n = int(self.lib.bhah_hip_MoL_step_tape_doubles(self.handle))
+        if n < 0:
+            raise BHaHError("patch has no tape (not a BSSN patch)")
+        return n
+
+    def MoL_step_forward_taped(self, y_n, dt: float, tape):
+        """One RK4 step of y_n in place, the step's stages recorded in tape (a float64 device
+        tensor of at least tape_doubles values)."""
+        if tape.numel() < self.tape_doubles or tape.dtype.itemsize != 8:
+            raise BHaHError(f"tape must hold {self.tape_doubles} float64 values")
+        _check(self.lib.bhah_hip_MoL_step_forward_taped(self.handle, _ptr(y_n), dt, _ptr(tape)))
+
+    def MoL_step_adjoint_taped(self, tape, dt: float, lam):
+        """lam (adjoint of the state after the taped step) <- adjoint of the state before it."""
+        if tape.numel() < self.tape_doubles or tape.dtype.itemsize != 8:
+            raise BHaHError(f"tape must hold {self.tape_doubles} float64 values")
+        _check(self.lib.bhah_hip_MoL_step_adjoint_taped(self.handle, _ptr(tape), dt, _ptr(lam)))
 
 
 class MoLState:

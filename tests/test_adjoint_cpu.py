@@ -31,8 +31,10 @@ def adj_exe():
     wd = tempfile.mkdtemp(prefix="kadj_")
     exe = os.path.join(wd, "adj")
     src = os.path.join(GEN, f"adjoint_{TAG}.hip")
+    # threaded emulation: the rhs adjoint's output groups meet at a workgroup barrier
     subprocess.run(["g++", "-std=c++17", "-O1", "-g", "-fsanitize=address", "-fno-omit-frame-pointer", "-w",
-                    f"-I{SHIM}", f'-DKERNEL_FILE="{src}"', f"-DTAG={TAG}", DRIVER, "-o", exe, "-lm"],
+                    "-DBHAH_CPU_THREADED", "-pthread", f"-I{SHIM}", f'-DKERNEL_FILE="{src}"', f"-DTAG={TAG}", DRIVER,
+                    "-o", exe, "-lm"],
                    check=True, capture_output=True)
     yield exe
     shutil.rmtree(wd, ignore_errors=True)

@@ -42,6 +42,65 @@ def test_step_adjoint_dot_product(hip):
     assert abs(lhs - rhs) < 1e-6 * abs(lhs), (lhs, rhs)
 
 
+def test_taped_step_equals_the_mol_step_and_the_recomputing_adjoint(hip):
+    """The taped forward step gives the MoL step's state bit for bit, and the adjoint read
+    from its tape equals the adjoint that recomputes the stages, bit for bit (same kernels,
+    same inputs)."""
+    import torch
+    from nrpy_amd.runtime import MoLState, Patch, PatchConfig
+
+    P = Patch(PatchConfig(Nxx=(24, 8, 4), grid_physical_size=10.0, eta=2.0), tuned=False)
+    xx = P.xx()
+    y = ID.brill_lindquist(1, xx, 10.0, 0.2, (0.5, 0.4), (0, 0, 0.5), (0, 0, -0.6))
+    y = ID.smooth_perturbation(y, xx, 5, amplitude=2e-2)
+    yd = torch.from_numpy(np.ascontiguousarray(y)).cuda()
+    P.apply_bcs_outerextrap_and_inner(yd)
+    S = MoLState(P, yd.clone())
+    S.step(P.dt)
+    tape = torch.full((P.tape_doubles,), float("nan"), dtype=torch.float64, device="cuda")
+    yt = yd.clone()
+    P.MoL_step_forward_taped(yt, P.dt, tape)
+    torch.cuda.synchronize()
+    assert torch.equal(yt, S.y_n)
+    nv = yd.numel()
+    # states and right-hand sides at every point; Ricci_eval writes RbarDD at interior points
+    assert torch.isfinite(tape[: 8 * nv]).all()
+    NG = P.NG
+    aux = tape[8 * nv:].view(4 * 6, *yd.shape[1:])
+    assert torch.isfinite(aux[:, NG:-NG, NG:-NG, NG:-NG]).all()
+    assert torch.equal(tape[:nv].view_as(yd), yd)  # the tape starts with y_n
+    rng = np.random.default_rng(5)
+    lam = torch.from_numpy(rng.standard_normal(y.shape)).cuda()
+    la, lb = lam.clone(), lam.clone()
+    P.MoL_step_adjoint(yd, P.dt, la)
+    P.MoL_step_adjoint_taped(tape, P.dt, lb)
+    torch.cuda.synchronize()
+    assert torch.equal(la, lb)
+    with pytest.raises(Exception):
+        P.MoL_step_adjoint_taped(tape[:-1], P.dt, lb)
+
+
+def test_checkpointed_gradient_equals_the_full_tape(hip):
+    """Both backward modes (all steps taped in the forward run, or K-step segments re-run from
+    checkpoints and taped) give the same gradient bit for bit: the segments re-run the same
+    kernels on the same states."""
+    import torch
+    from nrpy_amd.adjoint import LapseLossConfig, LapseLossProblem
+
+    cfg = LapseLossConfig(Nxx=(24, 6, 2), nsteps=11, window=5, checkpoint_every=4, n_sample=6)
+    full = LapseLossProblem(cfg)
+    assert full.full_tape_fits()
+    seg = LapseLossProblem(LapseLossConfig(**{**cfg.__dict__, "tape_budget_gib": 1e-6}))
+    assert not seg.full_tape_fits()
+    grads = []
+    for prob in (full, seg):
+        theta = torch.tensor([0.5, 0.45, 0.5, -0.55], dtype=torch.float64, requires_grad=True)
+        prob.loss(theta).backward()
+        grads.append(theta.grad.numpy().copy())
+    print("full tape", grads[0], "segments", grads[1])
+    assert np.array_equal(grads[0], grads[1]), grads
+
+
 def test_lapse_loss_gradient_matches_central_differences(hip):
     import torch
     from nrpy_amd.adjoint import LapseLossProblem
