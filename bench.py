@@ -314,7 +314,7 @@ def _cpu_model() -> str:
         return "unknown"
 
 
-def cpu_baseline(c: dict, min_seconds: float = 10.0, max_steps: int = 8) -> dict:
+def cpu_baseline(c: dict, min_seconds: float = 10.0, max_steps: int = 24) -> dict:
     """The oracle (plain-C restatement) on the host cores: RK4 steps of the same grid until
     about ``min_seconds`` of CPU work (at most ``max_steps``)."""
     import numpy as np

@@ -171,40 +171,9 @@ PROLOGUE = """  const int i0 = p.NG + (int)(blockIdx.x * blockDim.x + threadIdx.
 """
 
 
-def _slots_for_inputs(spec: specs.Spec, fd_order: int, inputs: Dict[str, str], table: SlotTable,
-                      skip_aux: bool) -> Tuple[List[str], Dict[str, int]]:
-    """Store lines writing each input adjoint to its slot(s); returns (lines, aux slot map)."""
-    centre, aux, derivs, rfm, params = spec.classify()
-    dmap = {d.symbol: d for d in derivs}
-    lines, aux_slots = [], {}
-    for n in sorted(inputs):
-        v = inputs[n]
-        if n in centre:
-            k = table.add(spec.gfs.index(n), [((0, 0, 0), sp.Integer(1), (0, 0, 0))])
-            lines.append(f"  S[{k} * Ntot + idx] = {v};")
-        elif n in aux:
-            k = table.add(-1, [])
-            aux_slots[n] = k
-            lines.append(f"  S[{k} * Ntot + idx] = {v};")
-        elif n in dmap:
-            d = dmap[n]
-            g = spec.gfs.index(d.gf)
-            if d.op == "dupD":
-                ku = table.add(g, _deriv_taps(d, fd_order, "up"))
-                kd = table.add(g, _deriv_taps(d, fd_order, "dn"))
-                c = f"(betaU{d.dirs[0]} > 0.0)"
-                lines.append(f"  S[{ku} * Ntot + idx] = {c} ? {v} : 0.0;")
-                lines.append(f"  S[{kd} * Ntot + idx] = {c} ? 0.0 : {v};")
-            else:
-                k = table.add(g, _deriv_taps(d, fd_order))
-                lines.append(f"  S[{k} * Ntot + idx] = {v};")
-        # rfm and parameters are constants of the step: no adjoint
-    return lines, aux_slots
-
-
 def _alloc_slots(spec: specs.Spec, fd_order: int, names: List[str], table: "SlotTable") -> Dict[str, tuple]:
-    """One slot per input symbol (two for an upwinded derivative: up, down), as
-    ``_slots_for_inputs`` allocates them; auxiliary inputs are recorded in table.aux_slots."""
+    """One slot per input symbol (two for an upwinded derivative: up, down, split by the sign of
+    beta^i at the point); auxiliary inputs are recorded in table.aux_slots."""
     centre, aux, derivs, rfm, params = spec.classify()
     dmap = {d.symbol: d for d in derivs}
     out: Dict[str, tuple] = {}
@@ -229,51 +198,43 @@ def _deriv_dir(name: str) -> int:
     return fd.parse_deriv(name).dirs[0]
 
 
-# Output groups of the rhs_eval adjoint.  A group's reverse sweep needs the forward
-# temporaries of its own outputs only, so each is a fraction of the whole sweep (which keeps
-# ~1100 values live and spills); the groups run concurrently.  betU's right-hand side contains
-# lambdaU's (the Gamma-driver), so the three are balanced as {betU}, {lambdaU, vetU, alpha},
-# {the rest}.
+# Output groups of the adjoint kernels.  A group's reverse sweep needs the forward temporaries
+# of its own outputs only, so each is a fraction of the whole sweep (which keeps over a thousand
+# values live and spills); the groups run side by side.  betU's right-hand side contains
+# lambdaU's (the Gamma-driver), so the rhs groups are {betU}, {lambdaU, vetU, alpha}, {the rest}.
 RHS_GROUPS = [("betU",), ("lambdaU", "vetU", "alpha"), ("aDD", "hDD", "cf", "trK")]
+RICCI_GROUPS = [("RbarDD00", "RbarDD01"), ("RbarDD02", "RbarDD11"), ("RbarDD12", "RbarDD22")]
 
 
-def _rhs_groups(rspec: specs.Spec) -> List[List[str]]:
-    names = [g for g, _ in rspec.outputs]
+def _groups(spec: specs.Spec, prefixes) -> List[List[str]]:
+    names = [g for g, _ in spec.outputs]
     groups = [[n for n in names if any(n == pre or (n.startswith(pre) and n[len(pre):].isdigit()) for pre in grp)]
-              for grp in RHS_GROUPS]
+              for grp in prefixes]
     if sorted(sum(groups, [])) != sorted(names):
-        raise ValueError(f"rhs adjoint groups {groups} do not partition the outputs {names}")
+        raise ValueError(f"adjoint groups {groups} do not partition the outputs {names}")
     return [g for g in groups if g]
 
 
-def adjoint_tu(v: specs.Variant, fd_order: int) -> str:
-    tag = kernel_name_tag(v, fd_order)
-    rats: Dict[str, str] = {}
-    pr = emit.DoublePrinter(rats)
-    table = SlotTable()
-
-    # ---- rhs_eval adjoint, split by output group.  A workgroup is G waves over the same 64
-    # points of an xx0 row, wave g running group g's reverse sweep (each on its own SIMD).
-    # Each wave leaves its input adjoints in LDS; after one barrier the sums over the groups,
-    # in group order, are stored to one slot per input, so the result is deterministic and
-    # the transposed stencils read each input's adjoint once.
-    rspec = specs.bssn_rhs_spec(v)
-    groups = _rhs_groups(rspec)
+def _grouped_adjoint(kname: str, params: str, spec: specs.Spec, groups: List[List[str]], lam: Dict[str, str],
+                     fd_order: int, pr, table: "SlotTable", prologue: str) -> Tuple[str, Dict[str, tuple]]:
+    """The adjoint of one pointwise kernel, split by output group.  A workgroup is G waves over
+    the same 64 points of an xx0 row, wave g running group g's reverse sweep (each wave on its
+    own SIMD).  Each wave leaves its input adjoints in LDS; after one barrier the sums over the
+    groups, in group order, are stored to one slot per input symbol (two for an upwinded
+    derivative, by the sign of beta^i), so the result is deterministic and the transposed
+    stencils read each input's adjoint once.  Returns (kernel source, {input: slots})."""
     G = len(groups)
-    lam = {g: f"lam[{bssn.EVOL_GFS.index(g)} * Ntot + idx]" for g, _ in rspec.outputs}
-    upw = _upwind_prologue(rspec, fd_order, rats, tables=True)
     swept = []
-    all_inputs: Dict[str, str] = {}
+    all_inputs: set = set()
     for gi, names in enumerate(groups):
-        outs = [(g, e) for g, e in rspec.outputs if g in names]
+        outs = [(g, e) for g, e in spec.outputs if g in names]
         fwd, rev, inputs = _reverse_sweep(outs, lam, pr, f"g{gi}t")
         used = set()
         for _, e in outs:
-            used |= {str(s) for s in e.free_symbols}
-        swept.append((names, _input_defs(rspec, fd_order, used), fwd, rev, inputs))
-        all_inputs.update({n: "" for n in inputs})
-    slot_of = _alloc_slots(rspec, fd_order, sorted(all_inputs), table)
-    aux_slots = {n: [k[0]] for n, k in slot_of.items() if k[0] in table.aux_slots}
+            used |= {str(x) for x in e.free_symbols}
+        swept.append((names, _input_defs(spec, fd_order, used), fwd, rev, inputs))
+        all_inputs |= set(inputs)
+    slot_of = _alloc_slots(spec, fd_order, sorted(all_inputs), table)
     contrib: Dict[int, List[int]] = {}  # slot -> LDS rows holding its partial sums, group order
     cases, row = [], 0
     for gi, (names, defs, fwd, rev, inputs) in enumerate(swept):
@@ -282,7 +243,7 @@ def adjoint_tu(v: specs.Variant, fd_order: int) -> str:
             if n not in slot_of:  # reference-metric values and parameters: constants of the step
                 continue
             ks, val = slot_of[n], inputs[n]
-            if len(ks) == 2:  # upwinded derivative: up / down slots by the sign of beta^i
+            if len(ks) == 2:
                 c = f"(betaU{_deriv_dir(n)} > 0.0)"
                 parts = [(ks[0], f"{c} ? {val} : 0.0"), (ks[1], f"{c} ? 0.0 : {val}")]
             else:
@@ -293,20 +254,17 @@ def adjoint_tu(v: specs.Variant, fd_order: int) -> str:
                 row += 1
         body = "\n".join(defs + fwd + rev + writes)
         cases.append(f"  case {gi}: {{ // {', '.join(names)}\n{body}\n  break;\n  }}")
-    nrows = row
-    store_cases = []
     slots = sorted(contrib)
+    store_cases = []
     for gi in range(G):
         lines = [f"    S[{k} * Ntot + idx] = " + " + ".join(f"red[{r} * 64 + lane]" for r in contrib[k]) + ";"
                  for j, k in enumerate(slots) if j % G == gi]
         store_cases.append(f"  case {gi}:\n" + "\n".join(lines) + "\n    break;")
-    adj_rhs = f"""
-// {G} output groups, one wave each over the same 64 points; {nrows} LDS rows of partial sums
+    src = f"""
+// {G} output groups, one wave each over the same 64 points; {row} LDS rows of partial sums
 extern "C" __global__ void __launch_bounds__({64 * G})
-adj_rhs_{tag}(const double *__restrict__ in, const double *__restrict__ aux, const double *__restrict__ lam,
-              double *__restrict__ S, const double *__restrict__ rfm0, const double *__restrict__ rfm1,
-              const bhah_kparams p) {{
-  __shared__ double red[{nrows} * 64];
+{kname}({params}) {{
+  __shared__ double red[{row} * 64];
   const int lane = (int)threadIdx.x, group = (int)threadIdx.y;
   const int i0r = p.NG + (int)blockIdx.x * 64 + lane;
   const bool active = i0r < p.Nx0 - p.NG;
@@ -315,7 +273,7 @@ adj_rhs_{tag}(const double *__restrict__ in, const double *__restrict__ aux, con
   const int i2 = p.NG + (int)blockIdx.z;
   const long s1 = p.s1, s2 = p.s2, Ntot = p.Ntot;
   const long idx = i0 + s1 * i1 + s2 * i2;
-{upw}
+{prologue}
   switch (group) {{
 {chr(10).join(cases)}
   default:
@@ -329,27 +287,37 @@ adj_rhs_{tag}(const double *__restrict__ in, const double *__restrict__ aux, con
     break;
   }}
 }}
-"""
-    # ---- Ricci_eval adjoint: output adjoints are the auxiliary slots written above
-    qspec = specs.ricci_spec(v)
-    qlam = {g: f"S[{aux_slots[g][0]} * Ntot + idx]" if g in aux_slots else "0.0" for g, _ in qspec.outputs}
-    qfwd, qrev, qinputs = _reverse_sweep(qspec.outputs, qlam, pr, "u")
-    qused = set()
-    for _, e in qspec.outputs:
-        qused |= {str(s) for s in e.free_symbols}
-    qdefs = _input_defs(qspec, fd_order, qused)
-    qstores, _ = _slots_for_inputs(qspec, fd_order, qinputs, table, skip_aux=True)
-    adj_ricci = f"""
-extern "C" __global__ void __launch_bounds__(256)
-adj_ricci_{tag}(const double *__restrict__ in, double *__restrict__ S, const double *__restrict__ rfm0,
-                const double *__restrict__ rfm1, const bhah_kparams p) {{
-{PROLOGUE}
-{chr(10).join(qdefs)}
-{chr(10).join(qfwd)}
-{chr(10).join(qrev)}
-{chr(10).join(qstores)}
+// workgroups of 64 points of one xx0 row x {G} output groups
+static void {kname}_dims(const bhah_kparams &p, dim3 &g, dim3 &b) {{
+  g = dim3((unsigned)((p.Nx0 - 2 * p.NG + 63) / 64), (unsigned)(p.Nx1 - 2 * p.NG), (unsigned)(p.Nx2 - 2 * p.NG));
+  b = dim3(64, {G});
 }}
 """
+    return src, slot_of
+
+
+def adjoint_tu(v: specs.Variant, fd_order: int) -> str:
+    tag = kernel_name_tag(v, fd_order)
+    rats: Dict[str, str] = {}
+    pr = emit.DoublePrinter(rats)
+    table = SlotTable()
+
+    # ---- rhs_eval adjoint, in output groups (see _grouped_adjoint)
+    rspec = specs.bssn_rhs_spec(v)
+    lam = {g: f"lam[{bssn.EVOL_GFS.index(g)} * Ntot + idx]" for g, _ in rspec.outputs}
+    upw = _upwind_prologue(rspec, fd_order, rats, tables=True)
+    adj_rhs, slot_of = _grouped_adjoint(
+        f"adj_rhs_{tag}", "const double *__restrict__ in, const double *__restrict__ aux, const double *__restrict__ lam,\n"
+        "    double *__restrict__ S, const double *__restrict__ rfm0, const double *__restrict__ rfm1, const bhah_kparams p",
+        rspec, _groups(rspec, RHS_GROUPS), lam, fd_order, pr, table, upw)
+    aux_slot = {n: k[0] for n, k in slot_of.items() if k[0] in table.aux_slots}
+    # ---- Ricci_eval adjoint: its output adjoints are the auxiliary slots written above
+    qspec = specs.ricci_spec(v)
+    qlam = {g: f"S[{aux_slot[g]} * Ntot + idx]" if g in aux_slot else "0.0" for g, _ in qspec.outputs}
+    adj_ricci, _ = _grouped_adjoint(
+        f"adj_ricci_{tag}", "const double *__restrict__ in, double *__restrict__ S, const double *__restrict__ rfm0,\n"
+        "    const double *__restrict__ rfm1, const bhah_kparams p",
+        qspec, _groups(qspec, RICCI_GROUPS), qlam, fd_order, pr, table, "")
     # ---- det(gammabar) enforcement adjoint, all points, in place on lam
     dspec = specs.detgamma_spec(v)
     dlam = {g: f"l_{g}" for g, _ in dspec.outputs}
@@ -441,14 +409,13 @@ adj_gather_{tag}(const double *__restrict__ S, double *__restrict__ lam_y, const
 extern "C" int adj_nslots_{tag}(void) {{ return {nslots}; }}
 extern "C" hipError_t launch_adj_rhs_{tag}(dim3 g, dim3 b, hipStream_t s, const double *in, const double *aux,
     const double *lam, double *S, const double *rfm0, const double *rfm1, bhah_kparams p) {{
-  // workgroups of 64 points of one xx0 row x {G} output groups
-  g = dim3((unsigned)((p.Nx0 - 2 * p.NG + 63) / 64), (unsigned)(p.Nx1 - 2 * p.NG), (unsigned)(p.Nx2 - 2 * p.NG));
-  b = dim3(64, {G});
+  adj_rhs_{tag}_dims(p, g, b);
   hipLaunchKernelGGL(adj_rhs_{tag}, g, b, 0, s, in, aux, lam, S, rfm0, rfm1, p);
   return hipGetLastError();
 }}
 extern "C" hipError_t launch_adj_ricci_{tag}(dim3 g, dim3 b, hipStream_t s, const double *in, double *S,
     const double *rfm0, const double *rfm1, bhah_kparams p) {{
+  adj_ricci_{tag}_dims(p, g, b);
   hipLaunchKernelGGL(adj_ricci_{tag}, g, b, 0, s, in, S, rfm0, rfm1, p);
   return hipGetLastError();
 }}
