@@ -929,9 +929,9 @@ def _group_bounds(lb: str, k: int, ngroups: int) -> str:
 # are no longer built (their timings stay in the tuned tables under nrpy_amd/autotune/).
 FULL_CANDIDATES = {
     "rhs": ["p1_lb1", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg12_ub_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg12_ub_mi_xcd",
-            "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg_ub_mi_xcd"],
+            "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg_ub_mi_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg_ub_mi_xcd_blk"],
     "ricci": ["r1_lb1", "r1_lb1_sj_L16x4x4x1_d2_dma_io_pg2_xcd", "r1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_mi_xcd",
-              "r1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg2_mi_xcd"],
+              "r1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg2_mi_xcd", "r1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg2_mi_xcd_blk"],
 }
 SMALL_CANDIDATES = {"rhs": ["p1_lb1", "p1_lb1_sl_lf_pf16_u32_w2", "p1_lb1_sj_L16x8x2x1_d3_dma_rb1",
                             "p1_lb1_sj_L16x8x2x1_d3_dma_rb1_pg8", "p1_lb1_sj_L16x8x2x1_d3_dma_rb1_pg8_ub",
@@ -969,17 +969,28 @@ class _LdsEmitter(StencilEmitter):
         return f"sh[{self.centre}{f' + {o}' if o > 0 else (f' - {-o}' if o < 0 else '')}]"
 
 
-def _lds_tile_order(xcd: bool, zfirst: bool) -> str:
+def _lds_tile_order(xcd: bool, zfirst: bool, blocked: bool = False) -> str:
     """Tile of this workgroup.  Workgroups are dispatched in linear order, round-robin
     over the 8 XCDs; with ``xcd`` XCD k takes the k-th contiguous eighth of the tiles
     (neighbouring tiles, whose halo boxes overlap, then share an L2), and with
     ``zfirst`` consecutive tiles run along xx2 first (their boxes overlap in 2H of
-    TZ + 2H planes)."""
-    if not (xcd or zfirst):
+    TZ + 2H planes).  With ``blocked`` each run of 32 consecutive tiles (an XCD's 32 CUs
+    hold one workgroup each) is a 2 x 4 x 4-tile block, so the tiles in flight together on
+    an XCD share their halo planes in every direction instead of along xx0 and xx1 only;
+    grids that do not divide into such blocks keep the linear order."""
+    if not (xcd or zfirst or blocked):
         return "  const unsigned bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;"
     lin = ("((n & 7u) == 0u) ? (h & 7u) * (n >> 3) + (h >> 3) : h" if xcd else "h")
     dec = ("bz = l % gz;\n    bx = (l / gz) % gx;\n    by = l / (gz * gx);" if zfirst else
            "bx = l % gx;\n    by = (l / gx) % gy;\n    bz = l / (gx * gy);")
+    if blocked:
+        dec = ("if ((gx & 1u) == 0u && (gy & 3u) == 0u && (gz & 3u) == 0u) {\n"
+               "      const unsigned w = l & 31u, b = l >> 5, nbx = gx >> 1, nby = gy >> 2;\n"
+               "      bx = ((b % nbx) << 1) + (w & 1u);\n"
+               "      by = (((b / nbx) % nby) << 2) + ((w >> 1) & 3u);\n"
+               "      bz = ((b / (nbx * nby)) << 2) + (w >> 3);\n"
+               "    } else {\n"
+               "      " + dec.replace("\n    ", "\n      ") + "\n    }")
     return f"""  unsigned bx, by, bz;
   {{
     const unsigned gx = gridDim.x, gy = gridDim.y, gz = gridDim.z, n = gx * gy * gz;
@@ -1082,7 +1093,7 @@ def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str],
                min_waves: int, scope: str, lv: List[str], xcd: bool = False,
                depth: int = 1, zfirst: bool = False, group: int = 1, dma: bool = False,
                one_barrier: bool = False, ensemble: str = "", interleave: bool = False,
-               pin_group: int = -1, branchless_upwind: bool = False) -> Tuple[str, str, int]:
+               pin_group: int = -1, branchless_upwind: bool = False, blocked: bool = False) -> Tuple[str, str, int]:
     """One workgroup-tiled kernel.  Each gridfunction with derivatives is staged, one at a
     time, from global memory into a halo box in LDS: ``group`` gridfunctions per
     barrier-separated phase, with the loads of the next ``depth`` phases in flight while
@@ -1387,7 +1398,7 @@ def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str],
 {kname}(const double *__restrict__ in, const double *__restrict__ aux, double *__restrict__ out,
         const double *__restrict__ rfm0, const double *__restrict__ rfm1, const bhah_kparams p) {{
   __shared__ double sh[{nslots * SLOT}];
-{_lds_tile_order(xcd, zfirst)}
+{_lds_tile_order(xcd, zfirst, blocked)}
 {tile_body}
 }}
 """
@@ -1463,7 +1474,7 @@ def candidate_tu(v: specs.Variant, fd_order: int, kind: str, cand: str, ensemble
             src, launch, fl = lds_kernel(sub, fd_order, kname, rats, lds, 2 if lb == "lb2" else 1, scope,
                                          [f"out[{outs.index(g)} * Ntot + idx]" for g, _ in sub.outputs], xcd=xcd,
                                          depth=next((int(o[1:]) for o in opts if o[0] == "d" and o[1:].isdigit()), 1),
-                                         zfirst="zf" in opts,
+                                         zfirst="zf" in opts, blocked="blk" in opts,
                                          group=next((int(o[1:]) for o in opts if o[0] == "g" and o[1:].isdigit()), 1),
                                          dma="dma" in opts, one_barrier="rb1" in opts, ensemble=ename,
                                          interleave="io" in opts,

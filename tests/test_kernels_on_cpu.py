@@ -81,8 +81,8 @@ def rel_err(a, b):
     return num / np.where(den > 0, den, 1.0)
 
 
-def _check(oracle_lib, tag, cands, tiles):
-    c = CASES[tag]
+def _check(oracle_lib, tag, cands, tiles, Nxx=None):
+    c = dict(CASES[tag], **({"Nxx": Nxx} if Nxx else {}))
     O = oracle_lib.OracleGrid(c["coord"], c["Nxx"], c["fd_order"], c["size"], 0.2, variant=c["variant"])
     y = ID.brill_lindquist(O.coord, O.xx, O.g.p0, O.g.p1)
     y = np.ascontiguousarray(ID.smooth_perturbation(y, O.xx, 3))
@@ -105,6 +105,17 @@ def _check(oracle_lib, tag, cands, tiles):
 @pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
 def test_spherical_fd4_schedules_on_cpu(oracle_lib):
     _check(oracle_lib, "Spherical_sym2_KO0_fd4", schedules("Spherical_sym2_KO0_fd4")[:2], [(64, 4), (32, 8)])
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
+def test_blocked_tile_order_on_cpu(oracle_lib):
+    """The blocked tile order (runs of 32 tiles as 2 x 4 x 4-tile blocks) only takes effect
+    when the tile grid divides into blocks: 32x16x16 points are 2 x 4 x 4 tiles of 16x4x4, so
+    every tile must be computed exactly once through the blocked mapping."""
+    tag = "SinhSpherical_symnone_KO1_fd6"
+    cands = [kc for kc in schedules(tag) if kc[1].endswith("_blk")]
+    assert len(cands) == 2, cands
+    _check(oracle_lib, tag, cands, [(64, 4)], Nxx=(32, 16, 16))
 
 
 @pytest.mark.slow
