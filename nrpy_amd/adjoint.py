@@ -163,9 +163,13 @@ class LapseLossProblem:
         import torch
 
         if self.full_tape_fits():
-            tape = torch.empty((self.cfg.nsteps, self.P.tape_doubles), dtype=torch.float64, device="cuda")
-            L, s = self._forward(theta, tape=tape)
-            return L, s, ("tape", tape)
+            try:
+                tape = torch.empty((self.cfg.nsteps, self.P.tape_doubles), dtype=torch.float64, device="cuda")
+            except torch.cuda.OutOfMemoryError:
+                tape = None  # the device is shared or smaller than the budget: checkpoints instead
+            if tape is not None:
+                L, s = self._forward(theta, tape=tape)
+                return L, s, ("tape", tape)
         ck: List = []
         L, s = self._forward(theta, checkpoints=ck)
         return L, s, ("checkpoints", ck)
