@@ -12,10 +12,13 @@ with det(gammabar) enforcement.  Its adjoint is assembled from:
   symbol (centre value, auxiliary Ricci value, FD derivative) gets an adjoint
   "slot" field.  Upwinded derivatives get two slots, split by the sign of
   beta^i at the point (the stencil choice is piecewise constant in beta, so it
-  contributes no derivative of its own).
+  contributes no derivative of its own).  Each kernel is split into output
+  groups that run as the waves of one workgroup and sum their input adjoints
+  through LDS in a fixed order (``_grouped_adjoint``).
 * ``adj_gather``: the transpose of every FD stencil as a gather: at each point
   q and gridfunction g, the sum over the slots of g's derivatives of
-  weight x slot(q - offset).  Table-driven (slot, gridfunction, taps).
+  weight x slot(q - offset), unrolled per gridfunction with literal offsets
+  and weights.
 * ``adj_det``: the transposed Jacobian of det(gammabar) enforcement, pointwise.
 
 Boundary-condition and RK-combination adjoints are hand-written in the host
@@ -160,15 +163,6 @@ def _input_defs(spec: specs.Spec, fd_order: int, names) -> List[str]:
         else:
             raise ValueError(n)
     return lines
-
-
-PROLOGUE = """  const int i0 = p.NG + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int i1 = p.NG + (int)(blockIdx.y * blockDim.y + threadIdx.y);
-  const int i2 = p.NG + (int)blockIdx.z;
-  if (i0 >= p.Nx0 - p.NG || i1 >= p.Nx1 - p.NG) return;
-  const long s1 = p.s1, s2 = p.s2, Ntot = p.Ntot;
-  const long idx = i0 + s1 * i1 + s2 * i2;
-"""
 
 
 def _alloc_slots(spec: specs.Spec, fd_order: int, names: List[str], table: "SlotTable") -> Dict[str, tuple]:
