@@ -202,6 +202,34 @@ def bench_setup() -> dict:
             torch.cuda.synchronize()
             out[f"{name}_{'device' if where else 'host'}_s"] = time.perf_counter() - t0
             del ps
+    # the ensemble up to its first step: set-up and Brill-Lindquist data of all 64 members, on
+    # the host (host loops, numpy data, one copy per member) or on the device (set-up kernels,
+    # the closed-form data kernel)
+    import numpy as np
+
+    from nrpy_amd import initial_data as ID
+    from nrpy_amd.runtime import Ensemble
+
+    mem = c4_members()
+    states = {}
+    for where in (False, True):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        E = Ensemble([dataclasses.replace(cf, setup_on_device=where) for cf in c4])
+        for P, st, m in zip(E.members, E.states, mem):
+            pos1, pos2 = (0.0, 0.0, m["d"] / 2), (0.0, 0.0, -m["d"] / 2)
+            if where:
+                P.initial_data_brill_lindquist(st.y_n, m["m1"], pos1, m["m2"], pos2)
+            else:
+                y = ID.brill_lindquist(1, P.xx(), 10.0, 0.2, (m["m1"], m["m2"]), pos1, pos2)
+                st.y_n.copy_(torch.from_numpy(np.ascontiguousarray(y)))
+            P.apply_bcs_outerextrap_and_inner(st.y_n)
+        torch.cuda.synchronize()
+        out[f"ensemble_64_members_to_first_step_{'device' if where else 'host'}_s"] = time.perf_counter() - t0
+        states[where] = [st.y_n.clone() for st in E.states]
+        E.close()
+    out["ensemble_initial_data_max_abs_diff"] = max(float((a - b).abs().max()) for a, b in zip(states[False],
+                                                                                               states[True]))
     return out
 
 
