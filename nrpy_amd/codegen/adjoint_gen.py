@@ -19,7 +19,9 @@ with det(gammabar) enforcement.  Its adjoint is assembled from:
   q and gridfunction g, the sum over the slots of g's derivatives of
   weight x slot(q - offset), unrolled per gridfunction with literal offsets
   and weights.
-* ``adj_det``: the transposed Jacobian of det(gammabar) enforcement, pointwise.
+* ``adj_final`` / ``adj_stage``: the RK combinations with det(gammabar) enforcement,
+  reversed pointwise (the transposed Jacobian of the enforcement at the combination,
+  then the scatter to y_n and the stage right-hand sides).
 
 Boundary-condition and RK-combination adjoints are hand-written in the host
 runtime (they are linear maps with index lists).
@@ -312,29 +314,58 @@ def adjoint_tu(v: specs.Variant, fd_order: int) -> str:
         f"adj_ricci_{tag}", "const double *__restrict__ in, double *__restrict__ S, const double *__restrict__ rfm0,\n"
         "    const double *__restrict__ rfm1, const bhah_kparams p",
         qspec, _groups(qspec, RICCI_GROUPS), qlam, fd_order, pr, table, "")
-    # ---- det(gammabar) enforcement adjoint, all points, in place on lam
+    # ---- the RK combinations with det(gammabar) enforcement, reversed, at every point.  A stage
+    # input is Y = D(u), u = y_n + sum_j c_j K_j (D acts on hDD only), so the adjoint of Y goes
+    # through the transposed Jacobian of D at u and is then scattered to y_n and the K_j.
+    # adj_final: the last combination (y_{n+1}, weights b_j): lamYN = J_D^T lam, lamK_j = b_j lamYN.
+    # adj_stage: stage input Y_s (weight a on K_{s-1}): lamYN += J_D^T lamY, lamK_{s-1} += a (...).
+    # The operations are those of the separate combination, Jacobian and update kernels, in the
+    # same order, so the results are theirs bit for bit.
     dspec = specs.detgamma_spec(v)
     dlam = {g: f"l_{g}" for g, _ in dspec.outputs}
     dfwd, drev, dinputs = _reverse_sweep(dspec.outputs, dlam, pr, "w")
     dcentre, _, _, drfm, _ = dspec.classify()
-    dlines = [f"  const double {n} = u[{bssn.EVOL_GFS.index(n)} * Ntot + idx];" for n in dcentre]
-    dlines += [f"  const double {n} = {RFM_EXPR[specs.RFM_SYMBOLS.index(n)]};" for n in drfm]
-    dlines += [f"  const double l_{g} = lam[{bssn.EVOL_GFS.index(g)} * Ntot + idx];" for g, _ in dspec.outputs]
-    dstore = [f"  lam[{bssn.EVOL_GFS.index(g)} * Ntot + idx] = {dinputs.get(g, '0.0')};" for g, _ in dspec.outputs]
-    adj_det = f"""
-extern "C" __global__ void __launch_bounds__(256)
-adj_det_{tag}(const double *__restrict__ u, double *__restrict__ lam, const double *__restrict__ rfm0,
-              const double *__restrict__ rfm1, const bhah_kparams p) {{
-  const int i0 = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    det_out = {bssn.EVOL_GFS.index(g): g for g, _ in dspec.outputs}
+    ngf = len(bssn.EVOL_GFS)
+
+    def det_body(u_expr, lam_src: str) -> List[str]:
+        lines = [f"  const double {n} = {u_expr(bssn.EVOL_GFS.index(n))};" for n in dcentre]
+        lines += [f"  const double {n} = {RFM_EXPR[specs.RFM_SYMBOLS.index(n)]};" for n in drfm]
+        lines += [f"  const double l_{g} = {lam_src}[{bssn.EVOL_GFS.index(g)} * Ntot + idx];" for g, _ in dspec.outputs]
+        lines += dfwd + drev
+        # the adjoint of every gridfunction after the transposed Jacobian
+        lines += [f"  const double v{k} = " + (dinputs.get(det_out[k], "0.0") if k in det_out
+                                              else f"{lam_src}[{k} * Ntot + idx]") + ";" for k in range(ngf)]
+        return lines
+
+    point = """  const int i0 = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const int i1 = (int)(blockIdx.y * blockDim.y + threadIdx.y);
   const int i2 = (int)blockIdx.z;
   if (i0 >= p.Nx0 || i1 >= p.Nx1) return;
-  const long Ntot = p.Ntot;
-  const long idx = i0 + p.s1 * i1 + p.s2 * i2;
-{chr(10).join(dlines)}
-{chr(10).join(dfwd)}
-{chr(10).join(drev)}
-{chr(10).join(dstore)}
+  const long Ntot = p.Ntot, nv = """ + str(ngf) + """ * Ntot;
+  const long idx = i0 + p.s1 * i1 + p.s2 * i2;"""
+    final_body = det_body(lambda k: (f"y[{k} * Ntot + idx] + b0 * K[{k} * Ntot + idx] + b1 * K[nv + {k} * Ntot + idx]"
+                                     f" + b2 * K[2 * nv + {k} * Ntot + idx] + b3 * K[3 * nv + {k} * Ntot + idx]"),
+                          "lam")
+    final_body += [f"  lamYN[{k} * Ntot + idx] = v{k};" for k in range(ngf)]
+    final_body += [f"  lamK[{j} * nv + {k} * Ntot + idx] = b{j} * v{k};" for j in range(4) for k in range(ngf)]
+    stage_body = det_body(lambda k: f"y[{k} * Ntot + idx] + a * K[{k} * Ntot + idx]", "lamY")
+    stage_body += [f"  lamYN[{k} * Ntot + idx] += v{k};\n  lamK[{k} * Ntot + idx] += a * v{k};" for k in range(ngf)]
+    adj_rk = f"""
+extern "C" __global__ void __launch_bounds__(256)
+adj_final_{tag}(const double *__restrict__ y, const double *__restrict__ K, double b0, double b1, double b2,
+                double b3, const double *__restrict__ lam, double *__restrict__ lamYN, double *__restrict__ lamK,
+                const double *__restrict__ rfm0, const double *__restrict__ rfm1, const bhah_kparams p) {{
+{point}
+{chr(10).join(final_body)}
+}}
+
+extern "C" __global__ void __launch_bounds__(256)
+adj_stage_{tag}(const double *__restrict__ y, const double *__restrict__ K, double a,
+                const double *__restrict__ lamY, double *__restrict__ lamYN, double *__restrict__ lamK,
+                const double *__restrict__ rfm0, const double *__restrict__ rfm1, const bhah_kparams p) {{
+{point}
+{chr(10).join(stage_body)}
 }}
 """
     # ---- transposed stencils, unrolled per gridfunction: lam_y[g](q) += sum over the slots k
@@ -413,9 +444,15 @@ extern "C" hipError_t launch_adj_ricci_{tag}(dim3 g, dim3 b, hipStream_t s, cons
   hipLaunchKernelGGL(adj_ricci_{tag}, g, b, 0, s, in, S, rfm0, rfm1, p);
   return hipGetLastError();
 }}
-extern "C" hipError_t launch_adj_det_{tag}(dim3 g, dim3 b, hipStream_t s, const double *u, double *lam,
-    const double *rfm0, const double *rfm1, bhah_kparams p) {{
-  hipLaunchKernelGGL(adj_det_{tag}, g, b, 0, s, u, lam, rfm0, rfm1, p);
+extern "C" hipError_t launch_adj_final_{tag}(dim3 g, dim3 b, hipStream_t s, const double *y, const double *K,
+    double b0, double b1, double b2, double b3, const double *lam, double *lamYN, double *lamK, const double *rfm0,
+    const double *rfm1, bhah_kparams p) {{
+  hipLaunchKernelGGL(adj_final_{tag}, g, b, 0, s, y, K, b0, b1, b2, b3, lam, lamYN, lamK, rfm0, rfm1, p);
+  return hipGetLastError();
+}}
+extern "C" hipError_t launch_adj_stage_{tag}(dim3 g, dim3 b, hipStream_t s, const double *y, const double *K, double a,
+    const double *lamY, double *lamYN, double *lamK, const double *rfm0, const double *rfm1, bhah_kparams p) {{
+  hipLaunchKernelGGL(adj_stage_{tag}, g, b, 0, s, y, K, a, lamY, lamYN, lamK, rfm0, rfm1, p);
   return hipGetLastError();
 }}
 extern "C" hipError_t launch_adj_gather_{tag}(dim3 g, dim3 b, hipStream_t s, const double *S, double *lam_y,
@@ -426,4 +463,4 @@ extern "C" hipError_t launch_adj_gather_{tag}(dim3 g, dim3 b, hipStream_t s, con
 """
     head = f"// ---- adjoint of the BSSN step: {v.coord}, symmetry axes '{v.symmetry_axes}', fd_order={fd_order}; " \
            f"{nslots} adjoint slots\n"
-    return head + consts + "\n" + adj_rhs + adj_ricci + adj_det + gather + undefs + "\n" + launch
+    return head + consts + "\n" + adj_rhs + adj_ricci + adj_rk + gather + undefs + "\n" + launch

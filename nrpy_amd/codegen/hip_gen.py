@@ -1869,9 +1869,9 @@ def generate(outdir: str) -> List[str]:
             f"launch_bssn_rk_det_{tag}, launch_bssn_detgamma_{tag}, launch_bssn_constraints_{tag}, "
             f"launch_bssn_initial_lambda_{tag}, "
             f"{f'launch_bssn_ens_rk_det_{tag}' if ens else 'nullptr'}, "
-            + (f"adj_nslots_{tag}, launch_adj_rhs_{tag}, launch_adj_ricci_{tag}, launch_adj_det_{tag}, "
-               f"launch_adj_gather_{tag}}},"
-               if (v, order) in adjoint_builds else "nullptr, nullptr, nullptr, nullptr, nullptr},")
+            + (f"adj_nslots_{tag}, launch_adj_rhs_{tag}, launch_adj_ricci_{tag}, launch_adj_final_{tag}, "
+               f"launch_adj_stage_{tag}, launch_adj_gather_{tag}}},"
+               if (v, order) in adjoint_builds else "nullptr, nullptr, nullptr, nullptr, nullptr, nullptr},")
         )
         if (v, order) in adjoint_builds:
             from . import adjoint_gen
@@ -1886,8 +1886,12 @@ def generate(outdir: str) -> List[str]:
                 f"const double *, double *, const double *, const double *, bhah_kparams);",
                 f'extern "C" hipError_t launch_adj_ricci_{tag}(dim3, dim3, hipStream_t, const double *, double *, '
                 f"const double *, const double *, bhah_kparams);",
-                f'extern "C" hipError_t launch_adj_det_{tag}(dim3, dim3, hipStream_t, const double *, double *, '
+                f'extern "C" hipError_t launch_adj_final_{tag}(dim3, dim3, hipStream_t, const double *, '
+                f"const double *, double, double, double, double, const double *, double *, double *, "
                 f"const double *, const double *, bhah_kparams);",
+                f'extern "C" hipError_t launch_adj_stage_{tag}(dim3, dim3, hipStream_t, const double *, '
+                f"const double *, double, const double *, double *, double *, const double *, const double *, "
+                f"bhah_kparams);",
                 f'extern "C" hipError_t launch_adj_gather_{tag}(dim3, dim3, hipStream_t, const double *, double *, '
                 f"bhah_kparams);",
             ]

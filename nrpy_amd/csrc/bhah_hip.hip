@@ -54,8 +54,10 @@ struct BssnKernels {
                         const double *, const double *, bhah_kparams);
   hipError_t (*adj_ricci)(dim3, dim3, hipStream_t, const double *, double *, const double *, const double *,
                           bhah_kparams);
-  hipError_t (*adj_det)(dim3, dim3, hipStream_t, const double *, double *, const double *, const double *,
-                        bhah_kparams);
+  hipError_t (*adj_final)(dim3, dim3, hipStream_t, const double *, const double *, double, double, double, double,
+                          const double *, double *, double *, const double *, const double *, bhah_kparams);
+  hipError_t (*adj_stage)(dim3, dim3, hipStream_t, const double *, const double *, double, const double *, double *,
+                          double *, const double *, const double *, bhah_kparams);
   hipError_t (*adj_gather)(dim3, dim3, hipStream_t, const double *, double *, bhah_kparams);
 };
 #include "gen/kernel_table.inc"
@@ -579,50 +581,11 @@ bc_inner_adjoint_kernel(const int *__restrict__ dst, const int *__restrict__ by_
   l[src_list[u]] += acc;
 }
 
-// out = y + c0 k0 + c1 k1 + c2 k2 + c3 k3 over n values (RK stage arguments).
-__global__ void __launch_bounds__(256)
-rk_combine_kernel(double *__restrict__ out, const double *__restrict__ y, const double *__restrict__ k, long n,
-                  double c0, double c1, double c2, double c3) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  out[t] = y[t] + c0 * k[t] + c1 * k[n + t] + c2 * k[2 * n + t] + c3 * k[3 * n + t];
-}
-
 // dst[0..n) = v
 __global__ void __launch_bounds__(256)
 fill_kernel(double *__restrict__ dst, double v, long n) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) dst[t] = v;
-}
-
-// dst += a * src
-__global__ void __launch_bounds__(256)
-axpy_kernel(double *__restrict__ dst, const double *__restrict__ src, double a, long n) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) dst[t] += a * src[t];
-}
-
-// dst[j n + t] = c_j src[t], j = 0..3
-__global__ void __launch_bounds__(256)
-scale4_kernel(double *__restrict__ dst, const double *__restrict__ src, long n, double c0, double c1, double c2,
-              double c3) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const double v = src[t];
-  dst[t] = c0 * v;
-  dst[n + t] = c1 * v;
-  dst[2 * n + t] = c2 * v;
-  dst[3 * n + t] = c3 * v;
-}
-
-// a += x; b += c x
-__global__ void __launch_bounds__(256)
-axpy2_kernel(double *__restrict__ a, double *__restrict__ b, const double *__restrict__ x, double c, long n) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const double v = x[t];
-  a[t] += v;
-  b[t] += c * v;
 }
 
 // out = a + b
@@ -2319,24 +2282,24 @@ static int enqueue_stages(bhah_patch *P, const StepTape &T, double *yrun, double
 }
 
 // Workspace of the reverse sweep, after the recomputed tape and y_run in P->d_adj:
-// u, lamK1..4, lamY, lamYN (7 nv) and the adjoint slots S (nslots * Ntot, zero at every
+// lamK1..4, lamY, lamYN (6 nv) and the adjoint slots S (nslots * Ntot, zero at every
 // non-interior point: filled once when the workspace is made, then written at interior points
 // only, every slot at every point, by adj_rhs and adj_ricci).
 struct AdjWork {
-  double *u, *lamK, *lamY, *lamYN, *S;
+  double *lamK, *lamY, *lamYN, *S;
 };
 
 static AdjWork adj_work(const bhah_patch *P) {
   const long nv = P->Ntot * P->ngfs;
   double *w = P->d_adj + tape_doubles(P) + nv;
-  return {w, w + nv, w + 5 * nv, w + 6 * nv, w + 7 * nv};
+  return {w, w + 4 * nv, w + 5 * nv, w + 6 * nv};
 }
 
 static int ensure_adj_workspace(bhah_patch *P) {
   if (P->d_adj) return BHAH_OK;
   const long N = P->Ntot, nv = N * P->ngfs;
   const int nslots = P->k->adj_nslots();
-  const size_t total = (size_t)tape_doubles(P) + (size_t)nv + 7 * (size_t)nv + (size_t)nslots * N;
+  const size_t total = (size_t)tape_doubles(P) + (size_t)nv + 6 * (size_t)nv + (size_t)nslots * N;
   HIP_TRY(hipMalloc((void **)&P->d_adj, total * sizeof(double)));
   dim3 g, b;
   launch_1d((long)nslots * N, g, b);
@@ -2360,15 +2323,9 @@ static int enqueue_reverse(bhah_patch *P, const StepTape &T, double dt, double *
   int rc;
   const double a_stage[4] = {0.5 * dt, 0.5 * dt, dt, 0.0};  // Y[s] = D(y_n + a_stage[s-1] K_s), s = 1..3
   const double b_final[4] = {dt / 6.0, dt / 3.0, dt / 3.0, dt / 6.0};
-  // y_{n+1} = D(y_n + sum b K): lam_u = J_D^T lam, then lam_K_j = b_j lam_u
-  hipLaunchKernelGGL(rk_combine_kernel, g1, b1, 0, st, W.u, y_n, K, nv, b_final[0], b_final[1], b_final[2],
-                     b_final[3]);
-  HIP_TRY(hipGetLastError());
-  HIP_TRY(hipMemcpyAsync(W.lamYN, lam, nv * sizeof(double), hipMemcpyDeviceToDevice, st));
-  HIP_TRY(P->k->adj_det(ga, ba, st, W.u, W.lamYN, P->d_rfm0, P->d_rfm1, P->kp(dt, 0)));
-  hipLaunchKernelGGL(scale4_kernel, g1, b1, 0, st, W.lamK, W.lamYN, nv, b_final[0], b_final[1], b_final[2],
-                     b_final[3]);
-  HIP_TRY(hipGetLastError());
+  // y_{n+1} = D(y_n + sum b K): lamYN = J_D^T lam, then lamK_j = b_j lamYN
+  HIP_TRY(P->k->adj_final(ga, ba, st, y_n, K, b_final[0], b_final[1], b_final[2], b_final[3], lam, W.lamYN, W.lamK,
+                          P->d_rfm0, P->d_rfm1, P->kp(dt, 0)));
   for (int s = 4; s >= 1; s--) {
     const double *in = T.Y[s - 1], *aux = T.AUX + (s - 1) * 6 * N;
     double *lk = W.lamK + (s - 1) * nv;
@@ -2384,14 +2341,9 @@ static int enqueue_reverse(bhah_patch *P, const StepTape &T, double dt, double *
       HIP_TRY(hipGetLastError());
       break;
     }
-    // Y[s-1] = D(u), u = y_n + a K_{s-1}
-    double c[4] = {0, 0, 0, 0};
-    c[s - 2] = a_stage[s - 2];
-    hipLaunchKernelGGL(rk_combine_kernel, g1, b1, 0, st, W.u, y_n, K, nv, c[0], c[1], c[2], c[3]);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(P->k->adj_det(ga, ba, st, W.u, W.lamY, P->d_rfm0, P->d_rfm1, P->kp(dt, 0)));
-    hipLaunchKernelGGL(axpy2_kernel, g1, b1, 0, st, W.lamYN, W.lamK + (s - 2) * nv, W.lamY, a_stage[s - 2], nv);
-    HIP_TRY(hipGetLastError());
+    // Y[s-1] = D(y_n + a K_{s-1}): lamYN += J_D^T lamY, lamK_{s-1} += a J_D^T lamY
+    HIP_TRY(P->k->adj_stage(ga, ba, st, y_n, K + (s - 2) * nv, a_stage[s - 2], W.lamY, W.lamYN,
+                            W.lamK + (s - 2) * nv, P->d_rfm0, P->d_rfm1, P->kp(dt, 0)));
   }
   return BHAH_OK;
 }

@@ -1,10 +1,11 @@
 /* TEST INFRASTRUCTURE ONLY.  Runs the generated adjoint kernels of one BSSN build
  * on the CPU.  Built with -DKERNEL_FILE="<gen adjoint .hip>" -DTAG=<build tag>.
  *
- * Input: int32 mode(0: rhs+Ricci+gather, 1: det) Nx0 Nx1 Nx2 NG; float64 invdxx0..2
+ * Input: int32 mode(0: rhs+Ricci+gather, 1: det via adj_stage, 2: det via adj_final) Nx0 Nx1 Nx2 NG;
+ * float64 invdxx0..2
  * eta KO_gauge KO_nongauge wavespeed; in[24*Ntot]; aux[6*Ntot]; lam[24*Ntot];
  * rfm0[4*Nx0]; rfm1[3*Nx1].  Output: lam_y[24*Ntot] (mode 0: transposed
- * rhs_eval o Ricci_eval applied to lam at interior points; mode 1: transposed
+ * rhs_eval o Ricci_eval applied to lam at interior points; modes 1, 2: transposed
  * det(gammabar) enforcement at u = in). */
 #include KERNEL_FILE
 #include <stdio.h>
@@ -49,9 +50,16 @@ int main(int argc, char **argv) {
     CAT(launch_adj_gather_, TAG)(g1, b1, 0, S, out, p);
     free(S);
   } else {
-    memcpy(out, lam, 24 * N * sizeof(double));
+    /* the RK/det adjoints with zero combination weights: u = in, so out = J_D(in)^T lam
+     * (mode 1 through adj_stage: lamYN += ..., mode 2 through adj_final: lamYN = ...) */
+    double *K = (double *)calloc(4 * 24 * N, sizeof(double)), *lamK = (double *)calloc(4 * 24 * N, sizeof(double));
     dim3 b(64, 4, 1), g((p.Nx0 + 63) / 64, (p.Nx1 + 3) / 4, p.Nx2);
-    CAT(launch_adj_det_, TAG)(g, b, 0, in, out, rfm0, rfm1, p);
+    if (h[0] == 1)
+      CAT(launch_adj_stage_, TAG)(g, b, 0, in, K, 0.0, lam, out, lamK, rfm0, rfm1, p);
+    else
+      CAT(launch_adj_final_, TAG)(g, b, 0, in, K, 0.0, 0.0, 0.0, 0.0, lam, out, lamK, rfm0, rfm1, p);
+    free(K);
+    free(lamK);
   }
   FILE *o = fopen(argv[2], "wb");
   fwrite(out, sizeof(double), 24 * N, o);

@@ -97,7 +97,10 @@ def test_rhs_ricci_adjoint_dot_product(oracle_lib, adj_exe, seed):
 
 
 @pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
-def test_det_enforcement_adjoint_dot_product(oracle_lib, adj_exe):
+@pytest.mark.parametrize("mode", [1, 2])
+def test_det_enforcement_adjoint_dot_product(oracle_lib, adj_exe, mode):
+    """Through both fused RK/det adjoints (1: adj_stage, 2: adj_final) with zero combination
+    weights, so their Jacobian part alone is checked."""
     O = oracle_lib.OracleGrid(1, (12, 8, 2), 4, 7.5, 0.2, variant=0)
     u = _state(O, 5)
     rng = np.random.default_rng(7)
@@ -105,6 +108,6 @@ def test_det_enforcement_adjoint_dot_product(oracle_lib, adj_exe):
     v = rng.standard_normal(u.shape)
     h = 1e-6
     lhs = np.sum(lam * (O.enforce_detgamma(u + h * v) - O.enforce_detgamma(u - h * v))) / (2 * h)
-    adj = _run(adj_exe, O, 1, u, np.zeros((6, *O.Nx[::-1])), lam)
+    adj = _run(adj_exe, O, mode, u, np.zeros((6, *O.Nx[::-1])), lam)
     rhs = np.sum(adj * v)
     assert abs(lhs - rhs) < 1e-7 * abs(lhs), (lhs, rhs)
