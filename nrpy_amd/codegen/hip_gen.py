@@ -929,7 +929,8 @@ def _group_bounds(lb: str, k: int, ngroups: int) -> str:
 # are no longer built (their timings stay in the tuned tables under nrpy_amd/autotune/).
 FULL_CANDIDATES = {
     "rhs": ["p1_lb1", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg12_ub_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg12_ub_mi_xcd",
-            "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg_ub_mi_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg_ub_mi_xcd_blk"],
+            "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg_ub_mi_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg_ub_mi_xcd_blk",
+            "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg_ub_sa_mi_xcd"],
     "ricci": ["r1_lb1", "r1_lb1_sj_L16x4x4x1_d2_dma_io_pg2_xcd", "r1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_mi_xcd",
               "r1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg2_mi_xcd", "r1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg2_mi_xcd_blk"],
 }
@@ -1075,6 +1076,53 @@ def _int_pair_sum(w: Dict[int, sp.Rational], D: int, pts: Dict[int, str]) -> str
     return f"{'-' if sgn < 0 else ''}({' + '.join(parts)})"
 
 
+def _sym_line_weights(w_up, w_dn, w_ko):
+    """Symmetric and antisymmetric parts of the one-sided upwind stencil along a line.
+
+    With a_j = f_j + f_-j and b_j = f_j - f_-j, S = 2 w_0 f_0 + sum_j (w_j + w_-j) a_j and
+    A = sum_j (w_j - w_-j) b_j (w the ``dupD`` weights): dupD = (S + A) / 2, and when the
+    ``ddnD`` stencil is the mirror image with the sign flipped, ddnD = (A - S) / 2.  For
+    NRPy's stencils the KO stencil is r S (upwinding is centred differencing plus exactly
+    this dissipation), so one S serves the upwind sum of either sign and the KO sum.
+    Returns (integer weights of S by j >= 0, of A by j >= 1, their common denominator DS,
+    r), or None when the stencils do not have this form."""
+    up = {k: sp.Rational(v) for k, v in w_up.items()}
+    dn = {k: sp.Rational(v) for k, v in w_dn.items()}
+    ko = {k: sp.Rational(v) for k, v in w_ko.items()}
+    R = max(abs(k) for k in list(up) + list(dn) + list(ko))
+
+    def g(w, k):
+        return w.get(k, sp.Integer(0))
+
+    if any(g(dn, k) != -g(up, -k) for k in range(-R, R + 1)) or any(g(ko, k) != g(ko, -k) for k in range(R + 1)):
+        return None
+    S = {0: 2 * g(up, 0), **{j: g(up, j) + g(up, -j) for j in range(1, R + 1)}}
+    A = {j: g(up, j) - g(up, -j) for j in range(1, R + 1)}
+    k0 = next(k for k in S if S[k] != 0)
+    r = g(ko, k0) / S[k0]
+    if any(g(ko, k) != r * S[k] for k in range(R + 1)):
+        return None
+    DS = int(sp.ilcm(*[v.q for v in list(S.values()) + list(A.values())]))
+    IS = {k: int(v * DS) for k, v in S.items() if v != 0}
+    IA = {k: int(v * DS) for k, v in A.items() if v != 0}
+    return IS, IA, DS, r
+
+
+def _int_line_sum(W: Dict[int, int], pts: Dict[int, str], mirror: str) -> Tuple[str, int]:
+    """(C expression, flops) of sum_j W[j] (pts[j] mirror pts[-j]) (W[0]: pts[0] alone),
+    outermost pair first, a weight of magnitude one leading as a plain (signed) value."""
+    parts, fl = [], 0
+    for o in sorted(W, reverse=True):
+        v = pts[0] if o == 0 else f"({pts[o]} {mirror} {pts[-o]})"
+        fl += 1 if o else 0
+        if not parts and abs(W[o]) == 1:
+            parts.append(v if W[o] == 1 else f"-{v}")
+        else:
+            parts.append(f"{W[o]}.0 * {v}")
+            fl += 2 if len(parts) > 1 else 1
+    return f"({' + '.join(parts)})", fl
+
+
 def is_lds(cand: str) -> bool:
     """Whether a schedule name selects the workgroup-tiled (LDS) kernels."""
     return any(o[:1] == "L" and o[1:2].isdigit() for o in cand.split("_"))
@@ -1093,7 +1141,8 @@ def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str],
                min_waves: int, scope: str, lv: List[str], xcd: bool = False,
                depth: int = 1, zfirst: bool = False, group: int = 1, dma: bool = False,
                one_barrier: bool = False, ensemble: str = "", interleave: bool = False,
-               pin_group: int = -1, branchless_upwind: bool = False, blocked: bool = False) -> Tuple[str, str, int]:
+               pin_group: int = -1, branchless_upwind: bool = False, blocked: bool = False,
+               sym_lines: bool = False) -> Tuple[str, str, int]:
     """One workgroup-tiled kernel.  Each gridfunction with derivatives is staged, one at a
     time, from global memory into a halo box in LDS: ``group`` gridfunctions per
     barrier-separated phase, with the loads of the next ``depth`` phases in flight while
@@ -1114,6 +1163,10 @@ def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str],
 
     Centred stencil sums are the same expressions, in the same order, as the global-load
     kernels; upwind and KO sums use integer weights with the denominator in a uniform scale.
+    With ``sym_lines`` a line carrying both an upwind and a KO term is formed from its
+    symmetric and antisymmetric sums (``_sym_line_weights``) with per-point coefficients
+    that fold in the upwind sign, the KO factor and the scales: two multiply-adds into the
+    output's accumulator and no select.
     Returns (kernel source, launch statement, flops per point)."""
     TX, TY, TZ, P = geom
     NT = TX * TY * TZ // P
@@ -1179,6 +1232,12 @@ def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str],
     w_up = dict(fd.weights_1d("dupD", fd_order))
     w_dn = dict(fd.weights_1d("ddnD", fd_order))
     w_ko = dict(fd.weights_1d("dKOD", fd_order))
+    sym = _sym_line_weights(w_up, w_dn, w_ko) if sym_lines and terms else None
+    if sym_lines and terms and sym is None:
+        raise ValueError(f"{kname}: the upwind and KO stencils have no shared symmetric part")
+    both = {key: cf for key, cf in line_cf.items() if "up" in cf and "ko" in cf} if sym else {}
+    ca_keys = sorted({(cf["up"], key[1]) for key, cf in both.items()})
+    csk_keys = sorted({(cf["up"], cf["ko"], key[1]) for key, cf in both.items()})
     L: List[str] = []
     a = L.append
     a(f"""  // Tile of {TX}x{TY}x{TZ} interior points; halo box {BX}x{BY}x{BZ} = {NB} values per gridfunction.
@@ -1227,6 +1286,16 @@ def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str],
             for d in range(3):
                 a(f"    L{k}_beta[{d}] = {pr.doprint(upw[d])};")
             a("  }")
+            if sym is not None:
+                # per line: acc += ca * A + csk * S, with ca = c_up * h / (2 DS) and
+                # csk = sign(beta) * ca + c_ko * h * r / DS (h = 1/dxx of the line's direction)
+                a(f"  double L{k}_ca[{len(ca_keys)}], L{k}_csk[{len(csk_keys)}];")
+                for i, (cu, d) in enumerate(ca_keys):
+                    a(f"  L{k}_ca[{i}] = L{k}_coef[{cu}] * (p.invdxx{d} * {_dbl(sp.Rational(1, 2 * sym[2]))});")
+                for m, (cu, ck, d) in enumerate(csk_keys):
+                    i = ca_keys.index((cu, d))
+                    a(f"  L{k}_csk[{m}] = ((L{k}_beta[{d}] > 0.0) ? L{k}_ca[{i}] : -L{k}_ca[{i}]) + "
+                      f"L{k}_coef[{ck}] * (p.invdxx{d} * {_dbl(sym[3] / sym[2])});")
             for g_out in sorted(terms):
                 a(f"  double V{k}_lin_{g_out} = 0.0;")
     phases = [staged[i:i + group] for i in range(0, len(staged), group)]
@@ -1274,6 +1343,16 @@ def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str],
                     cf = line_cf[key]
                     acc = f"V{k}_lin_{line_out[key]}"
                     pts = {o: st.load(g, tuple(o if r == dirn else 0 for r in range(3))) for o in range(-H, H + 1)}
+                    if key in both:
+                        S, fs = _int_line_sum(sym[0], pts, "+")
+                        A, fa = _int_line_sum(sym[1], pts, "-")
+                        i = ca_keys.index((cf["up"], dirn))
+                        m = csk_keys.index((cf["up"], cf["ko"], dirn))
+                        a(f"  {acc} = fma(L{k}_ca[{i}], {A}, fma(L{k}_csk[{m}], {S}, {acc}));")
+                        pin(acc)
+                        if k == 0:
+                            fl[0] += fs + fa + 4
+                        continue
                     tc = []
                     if "up" in cf:
                         # the sign taken out of each sum goes into its scale (a free operand
@@ -1478,7 +1557,7 @@ def candidate_tu(v: specs.Variant, fd_order: int, kind: str, cand: str, ensemble
                                          group=next((int(o[1:]) for o in opts if o[0] == "g" and o[1:].isdigit()), 1),
                                          dma="dma" in opts, one_barrier="rb1" in opts, ensemble=ename,
                                          interleave="io" in opts,
-                                         branchless_upwind="ub" in opts,
+                                         branchless_upwind="ub" in opts, sym_lines="sa" in opts,
                                          pin_group=next((int(o[2:] or 0) for o in opts if o[:2] == "pg" and o[2:].isdigit() or o == "pg"), -1))
             kernels.append(src)
             calls.append(launch)
