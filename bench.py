@@ -46,49 +46,17 @@ PEAK_HBM_GBS = 8000.0
 # the same counters on a plain 512 MiB device copy (FETCH_SIZE x2.000, WRITE_SIZE
 # x1.000); summary and commands in profiles/bssn_sinhspherical_192x96x96_hbm_traffic.md.
 MEASURED_TRAFFIC = {
+    ("SinhSpherical_symnone_KO1_fd6", (192, 96, 96), "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg_ub_sa_mi_xcd_blk"): {
+        "fetch_bytes": 2 * 643740 * 1024.0,
+        "write_bytes": 333022 * 1024.0,
+    },
     ("SinhSpherical_symnone_KO1_fd6", (192, 96, 96), "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg_ub_sa_mi_xcd"): {
         "fetch_bytes": 2 * 1039814 * 1024.0,
         "write_bytes": 345712 * 1024.0,
     },
-    ("SinhSpherical_symnone_KO1_fd6", (192, 96, 96), "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg_ub_mi_xcd_blk"): {
-        "fetch_bytes": 2 * 671801 * 1024.0,
-        "write_bytes": 388385 * 1024.0,
-    },
     ("SinhSpherical_symnone_KO1_fd6", (192, 96, 96), "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg_ub_mi_xcd"): {
         "fetch_bytes": 2 * 1054470 * 1024.0,
         "write_bytes": 373350 * 1024.0,
-    },
-    ("SinhSpherical_symnone_KO1_fd6", (192, 96, 96), "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg12_ub_xcd"): {
-        "fetch_bytes": 2 * 1089230 * 1024.0,
-        "write_bytes": 442447 * 1024.0,
-    },
-    ("SinhSpherical_symnone_KO1_fd6", (192, 96, 96), "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg8_xcd"): {
-        "fetch_bytes": 2 * 1134320 * 1024.0,
-        "write_bytes": 448779 * 1024.0,
-    },
-    ("SinhSpherical_symnone_KO1_fd6", (192, 96, 96), "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_xcd"): {
-        "fetch_bytes": 2 * 1200440 * 1024.0,
-        "write_bytes": 452411 * 1024.0,
-    },
-    ("SinhSpherical_symnone_KO1_fd6", (192, 96, 96), "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_xcd"): {
-        "fetch_bytes": 2 * 1312480 * 1024.0,
-        "write_bytes": 425957 * 1024.0,
-    },
-    ("SinhSpherical_symnone_KO1_fd6", (192, 96, 96), "p1_lb1_sj_L16x4x4x1_d2_dma_xcd"): {
-        "fetch_bytes": 2 * 1155350 * 1024.0,
-        "write_bytes": 417525 * 1024.0,
-    },
-    ("SinhSpherical_symnone_KO1_fd6", (192, 96, 96), "p1_lb1_sj_L16x4x4x1_d2_xcd"): {
-        "fetch_bytes": 2 * 1131060 * 1024.0,
-        "write_bytes": 441872 * 1024.0,
-    },
-    ("SinhSpherical_symnone_KO1_fd6", (192, 96, 96), "p2_lb1_sj_ln_pf16_u32"): {
-        "fetch_bytes": 2 * (1414830 + 1087680) * 1024.0,
-        "write_bytes": (207360 + 373248) * 1024.0,
-    },
-    ("SinhSpherical_symnone_KO1_fd6", (192, 96, 96), "p3_lb1_sl"): {
-        "fetch_bytes": 2 * (625803 + 833376 + 1026360) * 1024.0,
-        "write_bytes": (152064 + 96768 + 387072) * 1024.0,
     },
 }
 

@@ -928,11 +928,9 @@ def _group_bounds(lb: str, k: int, ngroups: int) -> str:
 # and their nearest alternatives.  Schedules the autotuner measured slower on every workload
 # are no longer built (their timings stay in the tuned tables under nrpy_amd/autotune/).
 FULL_CANDIDATES = {
-    "rhs": ["p1_lb1", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg12_ub_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg12_ub_mi_xcd",
-            "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg_ub_mi_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg_ub_mi_xcd_blk",
-            "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg_ub_sa_mi_xcd"],
-    "ricci": ["r1_lb1", "r1_lb1_sj_L16x4x4x1_d2_dma_io_pg2_xcd", "r1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_mi_xcd",
-              "r1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg2_mi_xcd", "r1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg2_mi_xcd_blk"],
+    "rhs": ["p1_lb1", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg_ub_mi_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg_ub_sa_mi_xcd",
+            "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg_ub_sa_mi_xcd_blk"],
+    "ricci": ["r1_lb1", "r1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg2_mi_xcd", "r1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg2_mi_xcd_blk"],
 }
 SMALL_CANDIDATES = {"rhs": ["p1_lb1", "p1_lb1_sl_lf_pf16_u32_w2", "p1_lb1_sj_L16x8x2x1_d3_dma_rb1",
                             "p1_lb1_sj_L16x8x2x1_d3_dma_rb1_pg8", "p1_lb1_sj_L16x8x2x1_d3_dma_rb1_pg8_ub",
