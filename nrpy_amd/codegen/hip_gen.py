@@ -24,6 +24,7 @@ coalesced 512-byte row.
 from __future__ import annotations
 
 import os
+import re
 from typing import Dict, List, Set, Tuple
 
 import sympy as sp
@@ -928,9 +929,9 @@ def _group_bounds(lb: str, k: int, ngroups: int) -> str:
 # and their nearest alternatives.  Schedules the autotuner measured slower on every workload
 # are no longer built (their timings stay in the tuned tables under nrpy_amd/autotune/).
 FULL_CANDIDATES = {
-    "rhs": ["p1_lb1", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg_ub_mi_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg_ub_sa_mi_xcd",
-            "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg_ub_sa_mi_xcd_blk"],
-    "ricci": ["r1_lb1", "r1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg2_mi_xcd", "r1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg2_mi_xcd_blk"],
+    "rhs": ["p1_lb1", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg_ub_mi_xcd", "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg_ub_sa_lr40_di_mi_xcd_blk",
+            "p1_lb1_sj_L16x4x4x1_d4_dma_rb1_io_pg_ub_sa_lr40_di_mi_xcd_blk"],
+    "ricci": ["r1_lb1", "r1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg2_mi_xcd", "r1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg_lr40_mi_xcd"],
 }
 SMALL_CANDIDATES = {"rhs": ["p1_lb1", "p1_lb1_sl_lf_pf16_u32_w2", "p1_lb1_sj_L16x8x2x1_d3_dma_rb1",
                             "p1_lb1_sj_L16x8x2x1_d3_dma_rb1_pg8", "p1_lb1_sj_L16x8x2x1_d3_dma_rb1_pg8_ub",
@@ -1121,6 +1122,49 @@ def _int_line_sum(W: Dict[int, int], pts: Dict[int, str], mirror: str) -> Tuple[
     return f"({' + '.join(parts)})", fl
 
 
+_SH_REF = re.compile(r"sh\[c\d+(?: [+-] \d+)?\]")
+
+
+def _batch_lds_reads(stmts: List[Tuple[str, List[str]]], nmax: int, prefix: str) -> List[Tuple[str, List[str]]]:
+    """Group a gridfunction's stencil statements by the LDS values they read: statements
+    reading few values (axis stencils, line sums) first, in their order, then the others
+    (mixed-derivative planes), packed greedily into batches of at most ``nmax`` distinct
+    values.  Each batch's values are read into registers and pinned there before its
+    statements, so the reads of a batch are in flight together and the batch waits on LDS
+    once.  Without this the compiler, short of registers, interleaves a few reads with the
+    arithmetic that uses them, and each phase is a chain of LDS round trips."""
+    small = [t for t in stmts if len(set(_SH_REF.findall(t[0]))) <= 9]
+    big = [t for t in stmts if len(set(_SH_REF.findall(t[0]))) > 9]
+    out: List[Tuple[str, List[str]]] = []
+    batch: List[Tuple[str, List[str]]] = []
+    refs: List[str] = []
+    nb = [0]
+
+    def flush():
+        if not batch:
+            return
+        names = {r: f"{prefix}_{nb[0]}_{i}" for i, r in enumerate(refs)}
+        nb[0] += 1
+        out.append(("".join(f"  double {n} = {r};\n" for r, n in names.items()).rstrip("\n"), []))
+        vals = list(names.values())
+        for i in range(0, len(vals), 24):
+            out.append((f"  BHAH_PINS({', '.join(f'{chr(34)}+v{chr(34)}({x})' for x in vals[i:i + 24])});", []))
+        for text, formed in batch:
+            out.append((_SH_REF.sub(lambda m: names[m.group(0)], text), formed))
+
+    for text, formed in small + big:
+        r = list(dict.fromkeys(_SH_REF.findall(text)))
+        new = [x for x in r if x not in refs]
+        if batch and len(refs) + len(new) > nmax:
+            flush()
+            batch, refs = [], []
+            new = r
+        refs += new
+        batch.append((text, formed))
+    flush()
+    return out
+
+
 def is_lds(cand: str) -> bool:
     """Whether a schedule name selects the workgroup-tiled (LDS) kernels."""
     return any(o[:1] == "L" and o[1:2].isdigit() for o in cand.split("_"))
@@ -1140,7 +1184,7 @@ def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str],
                depth: int = 1, zfirst: bool = False, group: int = 1, dma: bool = False,
                one_barrier: bool = False, ensemble: str = "", interleave: bool = False,
                pin_group: int = -1, branchless_upwind: bool = False, blocked: bool = False,
-               sym_lines: bool = False) -> Tuple[str, str, int]:
+               sym_lines: bool = False, read_batch: int = 0, dma_spread: bool = False) -> Tuple[str, str, int]:
     """One workgroup-tiled kernel.  Each gridfunction with derivatives is staged, one at a
     time, from global memory into a halo box in LDS: ``group`` gridfunctions per
     barrier-separated phase, with the loads of the next ``depth`` phases in flight while
@@ -1164,7 +1208,13 @@ def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str],
     With ``sym_lines`` a line carrying both an upwind and a KO term is formed from its
     symmetric and antisymmetric sums (``_sym_line_weights``) with per-point coefficients
     that fold in the upwind sign, the KO factor and the scales: two multiply-adds into the
-    output's accumulator and no select.
+    output's accumulator and no select.  With ``read_batch`` N a gridfunction's LDS values are
+    read in batches of at most N (the axis stencils first, then each mixed-derivative plane),
+    each batch pinned to registers before its arithmetic, so a batch costs one LDS round trip
+    (``_batch_lds_reads``).  With ``dma_spread`` the global-to-LDS loads a phase issues for a
+    later phase are spread through its statements: issued all at once after the barrier, the
+    four waves' loads queue for the address unit and each wave waits there before its first
+    LDS read.
     Returns (kernel source, launch statement, flops per point)."""
     TX, TY, TZ, P = geom
     NT = TX * TY * TZ // P
@@ -1304,8 +1354,12 @@ def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str],
         """LDS offset (doubles) of box q of phase ph."""
         return ((ph % depth) * group + q) * SLOT if dma else q * NB
 
-    def emit_phase(ph, gfs):
-        """Derivatives and linear-term contributions of the phase's gridfunctions."""
+    def emit_phase(ph, gfs, dma_lines=()):
+        """Derivatives and linear-term contributions of the phase's gridfunctions; with
+        ``dma_lines`` (the global-to-LDS loads of a later phase) those are spread evenly
+        through the phase's statements instead of all issued at its start."""
+        lines: List[str] = []
+        e = lines.append
         for q, g in enumerate(gfs):
             pins: List[str] = []
 
@@ -1317,23 +1371,22 @@ def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str],
                 # pin_group formed values (0: all of the staged gridfunction's), so that the
                 # LDS reads of several stencils can be in flight together.
                 if pin_group < 0:
-                    a(f"  BHAH_PIN({x});")
+                    e(f"  BHAH_PIN({x});")
                     return
                 if x not in pins:
                     pins.append(x)
                 if pin_group and len(pins) >= pin_group:
-                    a(f"  BHAH_PINS({', '.join(f'{chr(34)}+v{chr(34)}({y})' for y in pins)});")
+                    e(f"  BHAH_PINS({', '.join(f'{chr(34)}+v{chr(34)}({y})' for y in pins)});")
                     pins.clear()
 
             for k in range(P):
                 st = _LdsEmitter(sub, fd_order, f"c{k}", BX, BY, box(ph, q))
+                stmts: List[Tuple[str, List[str]]] = []  # (statement, values it forms)
                 if g in ovr and g in used:
-                    a(f"  double V{k}_{g} = {st.load(g, (0, 0, 0))};")
-                    pin(f"V{k}_{g}")
+                    stmts.append((f"  double V{k}_{g} = {st.load(g, (0, 0, 0))};", [f"V{k}_{g}"]))
                 for d in dused:
                     if d.gf == g:
-                        a(f"  double V{k}_{d.symbol} = {st.expr(d)};")
-                        pin(f"V{k}_{d.symbol}")
+                        stmts.append((f"  double V{k}_{d.symbol} = {st.expr(d)};", [f"V{k}_{d.symbol}"]))
                 for dirn in range(3):
                     key = (g, dirn)
                     if key not in line_out:
@@ -1346,8 +1399,7 @@ def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str],
                         A, fa = _int_line_sum(sym[1], pts, "-")
                         i = ca_keys.index((cf["up"], dirn))
                         m = csk_keys.index((cf["up"], cf["ko"], dirn))
-                        a(f"  {acc} = fma(L{k}_ca[{i}], {A}, fma(L{k}_csk[{m}], {S}, {acc}));")
-                        pin(acc)
+                        stmts.append((f"  {acc} = fma(L{k}_ca[{i}], {A}, fma(L{k}_csk[{m}], {S}, {acc}));", [acc]))
                         if k == 0:
                             fl[0] += fs + fa + 4
                         continue
@@ -1360,7 +1412,8 @@ def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str],
                         if branchless_upwind:
                             # both one-sided sums formed (from the points the KO sum reads
                             # anyway) and one selected: no branch splits the scheduling region
-                            a(f"  const double U{k}_{g}_{dirn} = {fu} * {su}, D{k}_{g}_{dirn} = {fd_} * {sd};")
+                            stmts.append((f"  const double U{k}_{g}_{dirn} = {fu} * {su}, D{k}_{g}_{dirn} = {fd_} * {sd};",
+                                          []))
                             tc.append(f"L{k}_coef[{cf['up']}] * ((L{k}_beta[{dirn}] > 0.0) ? U{k}_{g}_{dirn} : "
                                       f"D{k}_{g}_{dirn})")
                         else:
@@ -1372,10 +1425,32 @@ def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str],
                         tc.append(f"L{k}_coef[{cf['ko']}] * (ikos{dirn} * {_int_pair_sum(w_ko, DK, pts)})")
                         if k == 0:
                             fl[0] += 3 * (len(w_ko) // 2) + 3
-                    a(f"  {acc} += {' + '.join(tc)};")
-                    pin(acc)
+                    stmts.append((f"  {acc} += {' + '.join(tc)};", [acc]))
+                if read_batch:
+                    stmts = _batch_lds_reads(stmts, read_batch, f"R{k}_{g}")
+                for text, formed in stmts:
+                    e(text)
+                    for x in formed:
+                        pin(x)
             for i in range(0, len(pins), 24):
-                a(f"  BHAH_PINS({', '.join(f'{chr(34)}+v{chr(34)}({x})' for x in pins[i:i + 24])});")
+                e(f"  BHAH_PINS({', '.join(f'{chr(34)}+v{chr(34)}({x})' for x in pins[i:i + 24])});")
+        if dma_lines:
+            # load i goes where the statements' text passes (i + 1) / (loads + 1) of its
+            # length; every load is placed exactly once, in order
+            m = len(dma_lines)
+            tot = sum(len(ln) for ln in lines) or 1
+            merged: List[str] = []
+            cum, i = 0, 0
+            for ln in lines:
+                while i < m and cum >= (i + 1) * tot / (m + 1):
+                    merged.append(dma_lines[i])
+                    i += 1
+                merged.append(ln)
+                cum += len(ln)
+            merged += dma_lines[i:]
+            lines = merged
+        for ln in lines:
+            a(ln)
 
     if dma:
         # every lane of every wave issues NJ loads per box (padded boxes), so the number of
@@ -1393,6 +1468,13 @@ def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str],
     for (int j = 0; j < {NJ}; j++) BHAH_DMA16O(src, pofs[j], sh + {box(ph, q)} + 2 * (NT * j + 64 * wv));
   }}""")
 
+        def dma_lines(ph):
+            """The loads of phase ph one statement each, in the order dma_phase issues them."""
+            issue.append((ph, len(phases[ph]) * NJ))
+            return [f"  BHAH_DMA16O((const char *)(in + {sub.gfs.index(g)} * Ntot), pofs[{j}], "
+                    f"sh + {box(ph, q)} + 2 * (NT * {j} + 64 * wv));"
+                    for q, g in enumerate(phases[ph]) for j in range(NJ)]
+
         if one_barrier:
             # Refill, right after a phase's barrier, the slot the previous phase read: every
             # wave passed that barrier only after finishing the previous phase, so one barrier
@@ -1408,6 +1490,9 @@ def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str],
                 a(f"  // ---- {', '.join(gfs)}")
                 a(f"  BHAH_VMWAIT({later});")
                 a(f"  BHAH_BARRIER_PHASE({ph});")
+                if ph + depth - 1 < len(phases) and dma_spread:
+                    emit_phase(ph, gfs, dma_lines(ph + depth - 1))
+                    continue
                 if ph + depth - 1 < len(phases):
                     dma_phase(ph + depth - 1)
                 emit_phase(ph, gfs)
@@ -1556,6 +1641,8 @@ def candidate_tu(v: specs.Variant, fd_order: int, kind: str, cand: str, ensemble
                                          dma="dma" in opts, one_barrier="rb1" in opts, ensemble=ename,
                                          interleave="io" in opts,
                                          branchless_upwind="ub" in opts, sym_lines="sa" in opts,
+                                         read_batch=next((int(o[2:]) for o in opts if o[:2] == "lr" and o[2:].isdigit()), 0),
+                                         dma_spread="di" in opts,
                                          pin_group=next((int(o[2:] or 0) for o in opts if o[:2] == "pg" and o[2:].isdigit() or o == "pg"), -1))
             kernels.append(src)
             calls.append(launch)
