@@ -46,13 +46,9 @@ PEAK_HBM_GBS = 8000.0
 # the same counters on a plain 512 MiB device copy (FETCH_SIZE x2.000, WRITE_SIZE
 # x1.000); summary and commands in profiles/bssn_sinhspherical_192x96x96_hbm_traffic.md.
 MEASURED_TRAFFIC = {
-    ("SinhSpherical_symnone_KO1_fd6", (192, 96, 96), "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg_ub_sa_mi_xcd_blk"): {
-        "fetch_bytes": 2 * 643740 * 1024.0,
-        "write_bytes": 333022 * 1024.0,
-    },
-    ("SinhSpherical_symnone_KO1_fd6", (192, 96, 96), "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg_ub_sa_mi_xcd"): {
-        "fetch_bytes": 2 * 1039814 * 1024.0,
-        "write_bytes": 345712 * 1024.0,
+    ("SinhSpherical_symnone_KO1_fd6", (192, 96, 96), "p1_lb1_sj_L16x4x4x1_d4_dma_rb1_io_pg_ub_sa_lr40_di_mi_xcd_blk"): {
+        "fetch_bytes": 2 * 650252 * 1024.0,
+        "write_bytes": 346663 * 1024.0,
     },
     ("SinhSpherical_symnone_KO1_fd6", (192, 96, 96), "p1_lb1_sj_L16x4x4x1_d3_dma_rb1_io_pg_ub_mi_xcd"): {
         "fetch_bytes": 2 * 1054470 * 1024.0,

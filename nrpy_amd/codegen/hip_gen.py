@@ -1334,7 +1334,7 @@ def lds_kernel(sub: specs.Spec, fd_order: int, kname: str, rats: Dict[str, str],
             for d in range(3):
                 a(f"    L{k}_beta[{d}] = {pr.doprint(upw[d])};")
             a("  }")
-            if sym is not None:
+            if ca_keys:
                 # per line: acc += ca * A + csk * S, with ca = c_up * h / (2 DS) and
                 # csk = sign(beta) * ca + c_ko * h * r / DS (h = 1/dxx of the line's direction)
                 a(f"  double L{k}_ca[{len(ca_keys)}], L{k}_csk[{len(csk_keys)}];")
