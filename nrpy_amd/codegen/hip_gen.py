@@ -1801,6 +1801,80 @@ bssn_rk_det2_{tag}(const double *__restrict__ kin, const double *y_n, double *__
   for (int g = 0; g < NGF; g++) __builtin_nontemporal_store(nx2[g], &OUT2[g * S2]);
 }}
 """
+    # The same pass with the inner-parity boundary condition of the right-hand side folded in:
+    # at an inner ghost point k is the value at its source point with the parity sign
+    # (apply_bcs_inner_only__openmp.c:3-57).  kin is only read by this pass, so the value is the
+    # one the separate copy would have stored there: the same k, bit for bit.  Each pair of points
+    # reads its k from one 16-byte-aligned source pair (its own pair away from the inner ghost
+    # zones), so a gridfunction costs one 16-byte load, as in the pass above.
+    rkdet += f"""
+extern "C" __global__ void __launch_bounds__(256)
+bssn_rk_deti2_{tag}(const double *__restrict__ kin, const double *y_n, double *__restrict__ y_run, double *out,
+                    const double *__restrict__ rfm0, const double *__restrict__ rfm1, const bhah_kparams p,
+                    const int2 *__restrict__ pmap, const int *__restrict__ gfpar) {{
+  const long Ntot = p.Ntot;
+  const long q = (long)blockIdx.x * 256 + threadIdx.x;  // pair index
+  const long idx2 = q * 2;
+  if (idx2 >= Ntot) return;
+  constexpr int NGF = {ngf};
+  const bhah_d2 *K2 = (const bhah_d2 *)kin, *YN2 = (const bhah_d2 *)(y_n + idx2);
+  bhah_d2 *YR2 = (bhah_d2 *)(y_run + idx2), *OUT2 = (bhah_d2 *)(out + idx2);
+  const long S2 = Ntot / 2;  // gridfunction stride in pairs
+  // source pair; code bit 0 / bit 1: point 0 / point 1 takes the source pair's second value;
+  // bits 4-15 / 16-27: the parity-type masks of points 0 / 1 (bit t set: type t flips sign)
+  const int2 pm = pmap[q];
+  const long sp = pm.x;
+  const unsigned code = (unsigned)pm.y;
+#define BHAH_KPAIR(g)                                                                               \\
+  ({{                                                                                               \\
+    const bhah_d2 v_ = __builtin_nontemporal_load(&K2[(g) * S2 + sp]);                              \\
+    const unsigned par_ = (unsigned)gfpar[g];                                                       \\
+    const double a_ = (code & 1u) ? v_.y : v_.x, b_ = (code & 2u) ? v_.y : v_.x;                    \\
+    bhah_d2 k_;                                                                                     \\
+    k_.x = ((code >> (4 + par_)) & 1u) ? -a_ : a_;                                                  \\
+    k_.y = ((code >> (16 + par_)) & 1u) ? -b_ : b_;                                                 \\
+    k_;                                                                                             \\
+  }})
+  const double dt = p.dt;
+  bhah_d2 nx2[NGF];
+  if (p.substep == 1) {{
+#pragma unroll
+    for (int g = 0; g < NGF; g++) {{
+      const bhah_d2 k = BHAH_KPAIR(g);
+      __builtin_nontemporal_store((1.0 / 6.0) * dt * k, &YR2[g * S2]);
+      nx2[g] = (1.0 / 2.0) * dt * k + __builtin_nontemporal_load(&YN2[g * S2]);
+    }}
+  }} else if (p.substep == 4) {{
+#pragma unroll
+    for (int g = 0; g < NGF; g++) {{
+      const bhah_d2 k = BHAH_KPAIR(g);
+      nx2[g] = (1.0 / 6.0) * dt * k + __builtin_nontemporal_load(&YN2[g * S2]) + __builtin_nontemporal_load(&YR2[g * S2]);
+    }}
+  }} else {{
+    const double a = (p.substep == 2) ? (1.0 / 2.0) : 1.0;
+#pragma unroll
+    for (int g = 0; g < NGF; g++) {{
+      const bhah_d2 k = BHAH_KPAIR(g);
+      __builtin_nontemporal_store((1.0 / 3.0) * dt * k + __builtin_nontemporal_load(&YR2[g * S2]), &YR2[g * S2]);
+      nx2[g] = a * dt * k + __builtin_nontemporal_load(&YN2[g * S2]);
+    }}
+  }}
+#undef BHAH_KPAIR
+#pragma unroll
+  for (int h = 0; h < 2; h++) {{
+    const long idx = idx2 + h;
+    const int i0 = (int)(idx % p.Nx0), i1 = (int)((idx / p.Nx0) % p.Nx1);
+    double nxt[NGF];
+#pragma unroll
+    for (int g = 0; g < NGF; g++) nxt[g] = nx2[g][h];
+{det}
+#pragma unroll
+    for (int g = 0; g < NGF; g++) nx2[g][h] = nxt[g];
+  }}
+#pragma unroll
+  for (int g = 0; g < NGF; g++) __builtin_nontemporal_store(nx2[g], &OUT2[g * S2]);
+}}
+"""
     ens_rk = ""
     if ensemble:
         ens_rk = f"""
@@ -1896,6 +1970,19 @@ extern "C" hipError_t launch_bssn_rk_det_{tag}(dim3 g, dim3 b, hipStream_t s, co
   else
     hipLaunchKernelGGL(bssn_rk_det_{tag}, dim3((unsigned)((p.Ntot + 255) / 256)), dim3(256), 0, s, kin, y_n, y_run,
                        out, rfm0, rfm1, p);
+  return hipGetLastError();
+}}
+// The RK pass with the inner boundary condition folded in: pairs only (even Ntot, 16-byte aligned
+// arrays); a call that does not meet that returns hipErrorInvalidValue and launches nothing.
+extern "C" hipError_t launch_bssn_rk_det_inner_{tag}(hipStream_t s, const double *kin, const double *y_n,
+    double *y_run, double *out, const double *rfm0, const double *rfm1, bhah_kparams p, const int *pmap,
+    const int *gfpar) {{
+  if ((p.Ntot & 1) || ((reinterpret_cast<uintptr_t>(kin) | reinterpret_cast<uintptr_t>(y_n) |
+                        reinterpret_cast<uintptr_t>(y_run) | reinterpret_cast<uintptr_t>(out) |
+                        reinterpret_cast<uintptr_t>(pmap)) & 15))
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(bssn_rk_deti2_{tag}, dim3((unsigned)((p.Ntot / 2 + 255) / 256)), dim3(256), 0, s, kin, y_n, y_run,
+                     out, rfm0, rfm1, p, reinterpret_cast<const int2 *>(pmap), gfpar);
   return hipGetLastError();
 }}
 extern "C" hipError_t launch_bssn_constraints_{tag}(dim3 g, dim3 b, hipStream_t s, const double *in,
@@ -2030,7 +2117,8 @@ def generate(outdir: str) -> List[str]:
         table_rows.append(
             f"    {{{COORD_ENUM[v.coord]}, {sym_mask}, {int(v.enable_KO)}, {order}, \"{tag}\", "
             f"{arrays['rhs']}, {ncand['rhs']}, {arrays['ricci']}, {ncand['ricci']}, "
-            f"launch_bssn_rk_det_{tag}, launch_bssn_detgamma_{tag}, launch_bssn_constraints_{tag}, "
+            f"launch_bssn_rk_det_{tag}, launch_bssn_rk_det_inner_{tag}, launch_bssn_detgamma_{tag}, "
+            f"launch_bssn_constraints_{tag}, "
             f"launch_bssn_initial_lambda_{tag}, "
             f"{f'launch_bssn_ens_rk_det_{tag}' if ens else 'nullptr'}, "
             + (f"adj_nslots_{tag}, launch_adj_rhs_{tag}, launch_adj_ricci_{tag}, launch_adj_final_{tag}, "
@@ -2067,6 +2155,10 @@ def generate(outdir: str) -> List[str]:
         table_decl.append(
             f'extern "C" hipError_t launch_bssn_rk_det_{tag}(dim3, dim3, hipStream_t, const double *, const double *, '
             f"double *, double *, const double *, const double *, bhah_kparams);"
+        )
+        table_decl.append(
+            f'extern "C" hipError_t launch_bssn_rk_det_inner_{tag}(hipStream_t, const double *, const double *, '
+            f"double *, double *, const double *, const double *, bhah_kparams, const int *, const int *);"
         )
         table_decl.append(
             f'extern "C" hipError_t launch_bssn_detgamma_{tag}(dim3, dim3, hipStream_t, double *, const double *, '

@@ -11,6 +11,7 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <algorithm>
 #include <mutex>
@@ -43,6 +44,9 @@ struct BssnKernels {
   int n_ricci;
   hipError_t (*rk_det)(dim3, dim3, hipStream_t, const double *, const double *, double *, double *, const double *,
                        const double *, bhah_kparams);
+  // the same pass taking the inner-parity boundary condition of k with it (pairs of points only)
+  hipError_t (*rk_det_inner)(hipStream_t, const double *, const double *, double *, double *, const double *,
+                             const double *, bhah_kparams, const int *, const int *);
   hipError_t (*detgamma)(dim3, dim3, hipStream_t, double *, const double *, const double *, bhah_kparams);
   hipError_t (*constraints)(dim3, dim3, hipStream_t, const double *, const double *, double *, const double *,
                             const double *, bhah_kparams);
@@ -109,6 +113,14 @@ struct bhah_patch {
   double *d_rfm0 = nullptr, *d_rfm1 = nullptr, *d_xx0 = nullptr, *d_xx1 = nullptr, *d_xx2 = nullptr;
   int *d_inner_dst = nullptr, *d_inner_src = nullptr;
   unsigned short *d_inner_mask = nullptr;  // bit t set: parity type t flips sign
+  // per pair of points (2q, 2q+1): the 16-byte-aligned pair that holds the sources of both
+  // points' right-hand sides (the pair itself away from inner ghost points) and a code: bit 0 /
+  // bit 1 set when point 0 / point 1 takes that pair's second value, bits 4-15 / 16-27 the parity
+  // masks of points 0 / 1.  Null when some pair's sources are not in one aligned pair.
+  int *d_inner_pairs = nullptr;
+  // the MoL step takes the inner boundary condition of k inside the RK pass (environment
+  // BHAH_HIP_SEPARATE_INNER_BC=1 at patch creation keeps the separate copy, for comparison)
+  bool fold_inner = true;
   // inner points grouped by source: d_inner_by_src[d_inner_src_start[u] .. [u+1]) lists the
   // points (indices into the inner arrays) copied from source d_inner_src_list[u]
   int *d_inner_by_src = nullptr, *d_inner_src_start = nullptr, *d_inner_src_list = nullptr;
@@ -1583,6 +1595,31 @@ static int patch_upload(bhah_patch *P, const std::vector<double> &rfm0, const st
     bhah_hip_patch_destroy(P);
     return rc;
   }
+  if (d.system == BHAH_SYSTEM_BSSN && !(P->Ntot & 1) && !dst.empty()) {
+    std::vector<int> imap((size_t)P->Ntot, -1);
+    for (size_t p = 0; p < dst.size(); p++) imap[(size_t)dst[p]] = (int)p;
+    std::vector<int> pairs((size_t)P->Ntot);  // (source pair, code) per pair
+    bool ok = true;
+    for (long q = 0; q < P->Ntot / 2 && ok; q++) {
+      long sr[2];
+      unsigned m[2];
+      for (int e = 0; e < 2; e++) {
+        const int ip = imap[(size_t)(2 * q + e)];
+        sr[e] = ip >= 0 ? (long)src[(size_t)ip] : 2 * q + e;
+        m[e] = ip >= 0 ? mask[(size_t)ip] : 0u;
+        if (m[e] >> 12) ok = false;
+      }
+      if ((sr[0] >> 1) != (sr[1] >> 1)) ok = false;
+      pairs[(size_t)(2 * q)] = (int)(sr[0] >> 1);
+      pairs[(size_t)(2 * q + 1)] = (int)((sr[0] & 1) | ((sr[1] & 1) << 1) | (m[0] << 4) | (m[1] << 16));
+    }
+    if (ok && (rc = upload(pairs, &P->d_inner_pairs))) {
+      bhah_hip_patch_destroy(P);
+      return rc;
+    }
+    const char *sep = getenv("BHAH_HIP_SEPARATE_INNER_BC");
+    P->fold_inner = !(sep && sep[0] == '1');
+  }
   for (int gz = 0; gz < P->NG; gz++)
     for (int dd = 0; dd < 3; dd++)
       if ((rc = upload(P->outer[gz][dd], &P->d_outer[gz][dd]))) {
@@ -1637,6 +1674,7 @@ int bhah_hip_patch_destroy(bhah_patch *P) {
   hipFree(P->d_inner_src_list);
   hipFree(P->d_inner_src);
   hipFree(P->d_inner_mask);
+  hipFree(P->d_inner_pairs);
   hipFree(P->d_wavespeed);
   hipFree(P->d_finf);
   hipFree(P->d_red);
@@ -1781,8 +1819,8 @@ int bhah_hip_apply_bcs_outerextrap_and_inner(bhah_patch *P, double *gfs) {
   return bhah_hip_apply_bcs_inner_only(P, gfs);
 }
 
-int bhah_hip_apply_bcs_outerradiation_and_inner(bhah_patch *P, const double *gfs, double *rhs) {
-  if (!P || !gfs || !rhs) return fail(BHAH_ERR_ARG, "NULL argument");
+// The radiation condition on the outer points of rhs alone; the caller applies the inner one.
+static int apply_bcs_outerradiation(bhah_patch *P, const double *gfs, double *rhs) {
   if (P->d.coord == BHAH_CARTESIAN) return fail(BHAH_ERR_UNSUPPORTED, "radiation BCs implemented for spherical-like coordinates");
   const int R = (P->d.radiation_bc_fd_order > 0 ? P->d.radiation_bc_fd_order : 4) / 2;
   if (R < 1 || R > 3) return fail(BHAH_ERR_ARG, "radiation_bc_fd_order must be 2, 4 or 6");
@@ -1806,7 +1844,7 @@ int bhah_hip_apply_bcs_outerradiation_and_inner(bhah_patch *P, const double *gfs
                          P->d.grid_physical_size, P->d.SINHW, P->Nx[0], (long)P->Nx[0], (long)P->Nx[0] * P->Nx[1],
                          P->Ntot, P->invdxx[0]);
     HIP_TRY(hipGetLastError());
-    return bhah_hip_apply_bcs_inner_only(P, rhs);
+    return BHAH_OK;
   }
   for (int gz = 0; gz < P->NG; gz++)
     for (int d = 0; d < 3; d++) {
@@ -1819,6 +1857,13 @@ int bhah_hip_apply_bcs_outerradiation_and_inner(bhah_patch *P, const double *gfs
                          (long)P->Nx[0], (long)P->Nx[0] * P->Nx[1], P->Ntot, P->invdxx[0]);
       HIP_TRY(hipGetLastError());
     }
+  return BHAH_OK;
+}
+
+int bhah_hip_apply_bcs_outerradiation_and_inner(bhah_patch *P, const double *gfs, double *rhs) {
+  if (!P || !gfs || !rhs) return fail(BHAH_ERR_ARG, "NULL argument");
+  int rc;
+  if ((rc = apply_bcs_outerradiation(P, gfs, rhs))) return rc;
   return bhah_hip_apply_bcs_inner_only(P, rhs);
 }
 
@@ -1869,6 +1914,25 @@ static int enqueue_step(bhah_patch *P, bhah_MoL_gridfunctions *gf, double dt) {
   }
   // BSSN: Ricci -> RHS -> radiation/extrapolation + inner BCs on RHS -> RK update + det(gammabar)
   // enforcement in one pass (two_blackholes_collide.py:290-313).
+  const uintptr_t al = reinterpret_cast<uintptr_t>(gf->y_n_gfs) | reinterpret_cast<uintptr_t>(gf->k_odd_gfs) |
+                       reinterpret_cast<uintptr_t>(gf->k_even_gfs) |
+                       reinterpret_cast<uintptr_t>(gf->y_nplus1_running_total_gfs);
+  if (P->d.outer_bc == BHAH_OUTER_BC_RADIATION && P->fold_inner && P->k->rk_det_inner && P->d_inner_pairs &&
+      !(P->Ntot & 1) && !(al & 15)) {
+    // The inner copy of k rides in the RK pass, which gathers k at each inner ghost point from
+    // its source: k stays in k_even and every stage goes to k_odd, so the pass never writes
+    // the array it gathers from.  The same values reach every array as with the separate copy.
+    for (int s = 1; s <= 4; s++) {
+      double *in = (s == 1) ? gf->y_n_gfs : gf->k_odd_gfs, *k = gf->k_even_gfs;
+      double *next = (s == 4) ? gf->y_n_gfs : gf->k_odd_gfs;
+      if ((rc = bhah_hip_Ricci_eval(P, in, gf->auxevol_gfs))) return rc;
+      if ((rc = bhah_hip_rhs_eval(P, gf->auxevol_gfs, in, k))) return rc;
+      if ((rc = apply_bcs_outerradiation(P, in, k))) return rc;
+      HIP_TRY(P->k->rk_det_inner(P->stream, k, gf->y_n_gfs, gf->y_nplus1_running_total_gfs, next, P->d_rfm0,
+                                 P->d_rfm1, P->kp(dt, s), P->d_inner_pairs, P->d_gf_parity));
+    }
+    return BHAH_OK;
+  }
   for (int s = 1; s <= 4; s++) {
     double *in = in_s[s - 1], *k = out_s[s - 1];
     if ((rc = bhah_hip_Ricci_eval(P, in, gf->auxevol_gfs))) return rc;
